@@ -1,0 +1,145 @@
+"""Benchmark: FP64 sector H v on one MI355X at the 14-site half-filled sector (N_up = N_dn = 7,
+D = 3432^2 = 11,778,624), the reference's multiply_H (ed_hamiltonian.F90:88-191).
+
+One step = one H v over the whole sector, input and output resident in HBM.  Prints ONE JSON
+line with the algorithmic throughput 16*D bytes per H v (read x once, write y once) against the
+8 TB/s HBM peak, and the reference's own multiply_H (the reference build, oracle/_ref) timed on
+the host cores over MPI ranks as the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--config ns14_roofline]
+"""
+import argparse
+import json
+import math
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "mo-dmft_amd")]
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "H·v GB/s & % HBM peak at Ns=14 FP64; batched vs plain full-solve wall s"
+
+
+def cpu_reference_hv(cfg, nu, nd, calls=2):
+    """the reference's multiply_H (reference build) over MPI ranks on this host's cores"""
+    import numpy as np
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(drv) and os.path.exists(mpiexec)):
+        return None
+    ncores = min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<8i", 8, cfg.norb, cfg.nbath, nu, nd, calls, 0, 0))
+            f.write(struct.pack("<2d", cfg.U, cfg.J))
+            f.write(np.asarray(cfg.ek, "<f8").tobytes())
+            f.write(np.asarray(cfg.vk, "<f8").reshape(-1, order="F").tobytes())
+        r = subprocess.run([mpiexec, "-n", str(ncores), drv, fin, fout], capture_output=True, text=True,
+                           timeout=600, cwd=td)
+        if r.returncode != 0 or not os.path.exists(fout):
+            return None
+        raw = open(fout, "rb").read()
+        sec = struct.unpack("<d", raw[:8])[0]
+        ranks = struct.unpack("<i", raw[8:12])[0]
+    D = cfg.dim(nu, nd)
+    return {"value": 16.0 * D / sec / 1e9, "unit": "GB/s", "cores": ranks, "kind": "reference",
+            "sample": f"{calls} calls of the reference's multiply_H (ed_hamiltonian.F90:88, amdflang -O2 build of "
+                      f"the reference sources) on the same sector, {ranks} MPI ranks, one per host core; "
+                      f"{sec:.3f} s per H v", "seconds_per_hv": sec}
+
+
+def cpu_port_hv(cfg, nu, nd):
+    """fallback: the oracle's C restatement with OpenMP over rows"""
+    import numpy as np
+    import oracle
+    m = oracle.Model(cfg.norb, cfg.nbath, cfg.U, cfg.J, cfg.ek, cfg.vk, cfg.tij)
+    b = oracle.Basis(cfg.nsite, nu, nd)
+    x = np.random.default_rng(0).standard_normal(b.ntot)
+    nt = min(16, os.cpu_count() or 1)
+    t = time.time()
+    oracle.multiply_H(m, b, x, nt)
+    sec = time.time() - t
+    return {"value": 16.0 * b.ntot / sec / 1e9, "unit": "GB/s", "cores": nt, "kind": "port",
+            "sample": f"1 call of the C restatement of multiply_H, {nt} OpenMP threads; {sec:.3f} s per H v",
+            "seconds_per_hv": sec}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="ns14_roofline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this solver runs one sector on one GPU; multi-GPU runs are not part of this build "
+              "(the sharded H v design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import moed
+    from moed.model import config
+
+    cfg = config(args.config)
+    nu, nd = cfg.sectors[0]
+    solver = moed.EDSolver.from_config(cfg)
+    if args.threads or args.rows:
+        solver.set_hv_shape(args.threads, args.rows)
+    D = solver.dim(nu, nd)
+    dev = torch.device("cuda:0")
+    x = torch.empty(D, dtype=torch.float64, device=dev)
+    y = torch.empty(D, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = moed.lib().moed_fill_start_device(solver.h, 12345, D, x.data_ptr(), stream)
+    assert rc == 0
+    for _ in range(args.warmup):
+        solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record()
+    for _ in range(args.steps):
+        solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1) / args.steps
+    alg_bytes = 16.0 * D
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
+    out = {
+        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"H v, {cfg.nsite}-site sector N_up={nu} N_dn={nd}, D={D}",
+                   "norb": cfg.norb, "nbath": cfg.nbath, "U": cfg.U, "J": cfg.J, "sector": [nu, nd], "dim": D,
+                   "params": "seeded bath, tests/golden/configs.json"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "kernel": "hv_kernel"},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": round(wall, 4),
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(gbs / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

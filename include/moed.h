@@ -1,0 +1,160 @@
+/* moed.h -- C ABI of the MI355X exact-diagonalisation impurity solver (libmoed.so).
+ *
+ * The entry points replace the hot-path procedures of the reference DMFT-ED code
+ * (yw-choi/MO-DMFT); each declaration cites the reference interface it stands in for.
+ * The reference keeps its state in Fortran module globals (ed_config.f90:11-37); here the
+ * same state lives in an opaque handle created from moed_params.
+ *
+ * Conventions
+ *   - every pointer argument is a HOST pointer unless the name says _device;
+ *   - arrays are Fortran column-major, indices are int32 and 1-based where the reference's
+ *     are (orbital, spin, sector, level), vectors are 0-based C arrays of doubles laid out
+ *     as in the reference: element (iup, idown) at iup + nup*idown (ed_basis.F90:120-123);
+ *   - complex arrays are interleaved (re, im) doubles (Fortran double complex layout);
+ *   - every call is blocking (the stream is synchronised before return) except the _device
+ *     variants, which are ordered on the stream passed in;
+ *   - return value: MOED_OK (0) or a negative MOED_E* code; the message is available from
+ *     moed_last_error().  No exception crosses the ABI.
+ *   - one handle = one HIP stream = one host thread at a time (not re-entrant).
+ */
+#ifndef MOED_H
+#define MOED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOED_OK 0
+#define MOED_EARG -1     /* bad argument */
+#define MOED_ESECTOR -2  /* sector out of range or empty target sector */
+#define MOED_EZERO -3    /* zero start vector (c+|psi> = 0) */
+#define MOED_EHIP -4     /* HIP runtime error */
+#define MOED_ENOMEM -5   /* device allocation failed */
+#define MOED_ESTATE -6   /* call order: e.g. moed_green before moed_solve */
+#define MOED_ECONV -7    /* iteration limit reached (eigensolver, CG, Brent) */
+
+/* flags */
+#define MOED_F_EXACT_CMPLX 1u   /* evaluate G(iw), Delta(iw) at exact double E, w_n instead of the
+                                   reference's default-kind cmplx() (REAL*4-rounded) values */
+#define MOED_F_GF_REORTH 2u     /* full reorthogonalisation in the Green's-function Lanczos runs
+                                   (the reference has none, ed_lanczos.F90:37-68) */
+#define MOED_F_NO_TUNING 4u     /* ignore the launch-shape tuning table */
+
+typedef struct moed_handle moed_handle;
+
+/* Configuration: the reference's ed_config.f90 globals. */
+typedef struct moed_params {
+    int32_t norb;          /* Norb  (ed_config.f90:53) */
+    int32_t nsite;         /* Nsite = Norb + Nbath (ed_config.f90:55), <= 15 */
+    double U, J;           /* U, Jex (ed_config.f90:82-83) */
+    double beta;           /* beta (ed_config.f90:85) */
+    int32_t nstep;         /* Nstep: continued-fraction Lanczos steps (ed_config.f90:118) */
+    int32_t nev;           /* Nev: eigenpairs per sector (ed_config.f90:124) */
+    int32_t nw;            /* Nw: Matsubara frequencies (ed_config.f90:126) */
+    const double* ek;      /* ek(1:Nsite) */
+    const double* vk;      /* vk(1:Norb, Norb+1:Nsite), column-major */
+    const double* tij;     /* Norb x Norb impurity hopping, column-major, or NULL (not in reference) */
+    int32_t nsector;       /* Nsector and the Sectors block: nelec(:), nup(:) (ed_config.f90:66-80) */
+    const int32_t* nelec;
+    const int32_t* nup;
+    uint32_t flags;        /* MOED_F_* */
+    double gs_tol;         /* relative residual tolerance of the eigensolver (0 -> 1e-12) */
+    int32_t gs_maxvec;     /* Krylov basis cap of the eigensolver (0 -> 400) */
+    int32_t device;        /* HIP device ordinal */
+} moed_params;
+
+/* create / destroy.  Replaces the module initialisation ed_read_options + ed_green_init
+ * (ed_config.f90:43, ed_green.F90:25). */
+int moed_create(const moed_params* p, moed_handle** h);
+int moed_destroy(moed_handle* h);
+/* copies the last error message (NUL-terminated, truncated to len). */
+int moed_last_error(const moed_handle* h, char* buf, int32_t len);
+
+/* new bath (ek, vk), as the DMFT loop's mpi_bcast of ek/vk after the fit (main.f90:103-106). */
+int moed_set_bath(moed_handle* h, const double* ek, const double* vk);
+
+/* sector dimension and basis tables: generate_basis (ed_basis.F90:35-107).
+ * up_states[C(Nsite,nup_e)], dn_states[C(Nsite,ndn_e)] ascending; either may be NULL. */
+int moed_sector_dim(moed_handle* h, int32_t nup_e, int32_t ndn_e, int64_t* dim);
+int moed_basis(moed_handle* h, int32_t nup_e, int32_t ndn_e, int32_t* up_states, int32_t* dn_states);
+
+/* y = H x on sector (nup_e, ndn_e): multiply_H (ed_hamiltonian.F90:88-191). */
+int moed_hv(moed_handle* h, int32_t nup_e, int32_t ndn_e, const double* x, double* y);
+/* same on device pointers, ordered on `stream` (hipStream_t, NULL = the handle's stream). */
+int moed_hv_device(moed_handle* h, int32_t nup_e, int32_t ndn_e, const double* x_device, double* y_device,
+                   void* stream);
+
+/* vec_out = c+_{iorb,ispin} vec (pm=1) or c_{iorb,ispin} vec (pm=2) into the N+-1 sector:
+ * apply_c (ed_operators.F90:10-61).  y has the target sector's dimension. */
+int moed_apply_c(moed_handle* h, int32_t pm, int32_t iorb, int32_t ispin, int32_t nup_e, int32_t ndn_e,
+                 const double* x, double* y);
+
+/* lanczos_iteration (ed_lanczos.F90:14-117): a(0:nstep), b(0:nstep) with b = squared norms,
+ * b(0) = 0, breakdown when b(j) < 1.e-12 (REAL*4 literal) sets nstep_calc = j. */
+int moed_lanczos(moed_handle* h, int32_t nup_e, int32_t ndn_e, const double* vec, int32_t nstep,
+                 int32_t* nstep_calc, double* a, double* b);
+
+/* ed_solve (ed_solver.F90:28-105): lowest nev eigenpairs of every sector (diag, :107-191),
+ * SORT + boltzmann_factor, keep p > 1e-3 among the lowest nev, renormalise.
+ * Outputs have room for nev entries; sector and level are 1-based.  Eigenvectors stay on
+ * the device in the handle for moed_green.  Unfilled eigenvalues are DBL_MAX (the reference's
+ * huge(1d0) for unconverged pairs). */
+int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int32_t* sector,
+               int32_t* level);
+/* eigenvalues of one sector after moed_solve: ev[nev] (eigval_sec of ed_solver.F90:53). */
+int moed_sector_eigenvalues(moed_handle* h, int32_t isector, double* ev);
+/* copy eigenvector (isector, ilevel) to the host: import_eigvec (ed_io.F90:105-139). */
+int moed_eigvec(moed_handle* h, int32_t isector, int32_t ilevel, double* vec);
+
+/* ed_calc_green_ftn (ed_green.F90:60-134) over the states kept by moed_solve:
+ * Gr(Norb, nw) complex, accumulated as in green_diag (:136-185) at w = omega(1:nw). */
+int moed_green(moed_handle* h, int32_t nw, const double* omega, double* Gr);
+/* Lanczos records written to apbpanbn.dat by ed_calc_green_ftn (ed_green.F90:112-129):
+ * number of records, then record irec (0-based): iev, iorb, ispin, E, p, even, nsp, nsn and
+ * ap,bp,an,bn (room for nstep+1 each). */
+int moed_green_nrecords(moed_handle* h, int32_t* nrec);
+int moed_green_record(moed_handle* h, int32_t irec, int32_t* iev, int32_t* iorb, int32_t* ispin, double* E,
+                      double* p, int32_t* even, int32_t* nsp, int32_t* nsn, double* ap, double* bp, double* an,
+                      double* bn);
+
+/* continued fraction of green_diag (ed_green.F90:160-168 sign=+1; :176-184 sign=-1) for one
+ * coefficient set; g[nw] complex. */
+int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
+                  int32_t nw, const double* omega, double* g);
+/* adjoint of moed_green_cf: given gbar = dL/dRe g + i dL/dIm g (complex[nw]), returns
+ * ga[j] = dL/da_j, gb[j] = dL/db_j for j = 0..nsc.  No reference counterpart (north-star
+ * differentiation of G(iw) with respect to the Lanczos coefficients). */
+int moed_green_cf_adjoint(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a,
+                          const double* b, int32_t nw, const double* omega, const double* gbar, double* ga,
+                          double* gb);
+
+/* calc_dev (ed_green.F90:260-273): Dev(Norb, nw) complex from (ek, vk) and impurity levels ef. */
+int moed_calc_dev(moed_handle* h, const double* ek, const double* vk, const double* ef, int32_t nw,
+                  const double* omega, double* Dev);
+
+/* batched FUNC + dfunc (func.f90:1-142) for nfit parameter vectors x[nfit][nx] (nx = Nbath*(1+Norb)+Norb,
+ * packed as ev_to_x, func.f90:165-191), one shared target Dev(Norb, nw): f[nfit], g[nfit][nx]. */
+int moed_chi2_grad(moed_handle* h, int32_t nfit, int32_t nx, const double* x, int32_t nw, const double* omega,
+                   const double* Dev, double* f, double* g);
+/* minimization (func.f90:144-163 -> frprmn.f90) for nfit independent starting points at once:
+ * x[nfit][nx] in/out, fret[nfit], iter[nfit], status[nfit] (0 ok, MOED_ECONV on itmax). */
+int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t nw, const double* omega,
+                  const double* Dev, double ftol, double* fret, int32_t* iter, int32_t* status);
+
+/* Matsubara grid of ed_green_init (ed_green.F90:45-51) at one rank. */
+int moed_omega_grid(int32_t nw, double beta, double* omega);
+
+/* counter-based start vector x_i = 2*u(splitmix64(seed ^ i)) - 1 on the device. */
+int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x_device, void* stream);
+
+/* launch shape of the H v kernel: threads per block and rows per block (0 = tuned/default). */
+int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t rows);
+/* device pointer of the handle's stream (hipStream_t). */
+void* moed_stream(moed_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

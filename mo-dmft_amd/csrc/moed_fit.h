@@ -1,0 +1,32 @@
+// Bath fit on the device: batched chi^2 and gradient (func.f90:1-142) and the conjugate-gradient
+// minimiser (frprmn.f90) run for many independent starting points in lockstep.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+struct moed_handle;
+
+namespace moed {
+
+struct FitCtx {
+    hipStream_t stream;
+    int norb, nbath, compat;
+    std::string* err;
+};
+
+// f[nfit], g[nfit*nx] (either may be null); x[nfit*nx]; Dev interleaved complex (norb, nw) column-major
+int fit_eval(const FitCtx& c, int nfit, int nx, const double* x, int nw, const double* omega, const double* Dev,
+             double* f, double* g);
+// Delta(iw) (norb, nw) interleaved complex for one parameter vector x
+int fit_delta(const FitCtx& c, int nx, const double* x, int nw, const double* omega, double* Dev);
+int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
+                 double ftol, double* fret, int* iter, int* status);
+
+}  // namespace moed
+
+int fit_delta_host(moed_handle* h, int nx, const double* x, int nw, const double* omega, double* Dev);
+int fit_eval_host(moed_handle* h, int nfit, int nx, const double* x, int nw, const double* omega, const double* Dev,
+                  double* f, double* g);
+int fit_minimize_host(moed_handle* h, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
+                      double ftol, double* fret, int* iter, int* status);

@@ -1,0 +1,1121 @@
+// Host side of libmoed: the handle (the reference's ed_config.f90 module state), the per-popcount
+// string tables, the Lanczos drivers, the sector eigensolver that replaces PARPACK's diag
+// (ed_solver.F90:107-191), ed_solve / ed_calc_green_ftn drivers and the extern "C" ABI of moed.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "moed_fit.h"
+#include "moed_internal.h"
+#include "tridiag.h"
+
+using namespace moed;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, std::max<size_t>(b, 256));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = std::max<size_t>(b, 256);
+        return hipSuccess;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+int64_t binom(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    int64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// sign parity of moving one electron between bits p and q of string s: popcount of the bits strictly between
+inline int move_parity(uint32_t s, int p, int q) {
+    const int lo = std::min(p, q), hi = std::max(p, q);
+    const uint32_t mask = ((1u << hi) - 1u) & ~((1u << (lo + 1)) - 1u);
+    return __builtin_popcount(s & mask) & 1;
+}
+
+struct StrTable {
+    int n = 0, count = 0, maxhop = 0;
+    std::vector<uint32_t> str;
+    std::vector<int32_t> rank;  // [2^Ns]: rank within this popcount class, -1 otherwise
+    DevBuf d_str, d_onsite, d_nhop, d_hop, d_imp;
+    StrDev dev() const {
+        return {d_str.as<uint16_t>(), d_onsite.as<double>(), d_nhop.as<uint8_t>(), d_hop.as<uint32_t>(),
+                d_imp.as<int32_t>(), count, maxhop};
+    }
+};
+
+struct SectorEig {
+    int nu = 0, nd = 0;
+    int64_t D = 0;
+    int nfound = 0;
+    std::vector<double> ev;  // nev values, DBL_MAX where none
+    DevBuf vec;              // nfound vectors of D doubles
+};
+
+struct GreenRecord {
+    int iev, iorb, ispin;
+    double E, p;
+    int even, nsp, nsn;
+    std::vector<double> ap, bp, an, bn;
+};
+
+}  // namespace
+
+struct moed_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int norb = 0, nsite = 0, nbath = 0;
+    double U = 0, J = 0, beta = 0, UmJ3 = 0, Um2J = 0;
+    int nstep = 0, nev = 0, nw = 0;
+    uint32_t flags = 0;
+    double gs_tol = 1e-12;
+    int gs_maxvec = 400;
+    std::vector<double> ek, vk, tij;
+    bool has_tij = false;
+    std::vector<int> nelec, nup;
+    std::string err;
+    std::vector<double> coef;
+    DevBuf d_ek, d_coef;
+    std::map<int, std::unique_ptr<StrTable>> tables;
+    std::vector<std::unique_ptr<SectorEig>> eig;
+    bool solved = false;
+    std::vector<double> kept_val, kept_prob;
+    std::vector<int> kept_sector, kept_level;
+    std::vector<GreenRecord> records;
+    int hv_threads = 0, hv_rows = 0;
+    DevBuf ws[8];
+};
+
+namespace {
+
+enum { WS_X = 0, WS_Y, WS_PART, WS_SCAL, WS_V, WS_W, WS_MISC, WS_MISC2 };
+
+int fail(moed_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+int hipfail(moed_handle* h, hipError_t e, const char* what) {
+    return fail(h, e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                   \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return hipfail(h, _e, #expr); \
+    } while (0)
+
+// U - 3*J and U - 2*J rounded exactly as the reference forms them (ed_operators.F90:159,162):
+// the product first, then the difference, no fused multiply-add.
+void interaction_constants(double U, double J, double* umj3, double* um2j) {
+#pragma clang fp contract(off)
+    *umj3 = U - 3.0 * J;
+    *um2j = U - 2.0 * J;
+}
+
+int set_model(moed_handle* h, const double* ek, const double* vk) {
+    h->ek.assign(ek, ek + h->nsite);
+    h->vk.assign(vk, vk + (size_t)h->norb * h->nbath);
+    for (double v : h->ek)
+        if (!std::isfinite(v)) return fail(h, MOED_EARG, "ek contains a non-finite value");
+    for (double v : h->vk)
+        if (!std::isfinite(v)) return fail(h, MOED_EARG, "vk contains a non-finite value");
+    h->coef.assign((size_t)h->norb * h->nbath + (size_t)h->norb * h->norb, 0.0);
+    for (size_t i = 0; i < h->vk.size(); ++i) h->coef[i] = h->vk[i];
+    if (h->has_tij)
+        for (int i = 0; i < h->norb * h->norb; ++i) h->coef[(size_t)h->norb * h->nbath + i] = h->tij[i];
+    HIPCHK(h->d_ek.ensure(16 * sizeof(double)));
+    std::vector<double> ek16(16, 0.0);
+    std::copy(h->ek.begin(), h->ek.end(), ek16.begin());
+    HIPCHK(hipMemcpy(h->d_ek.p, ek16.data(), 16 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h->d_coef.ensure(h->coef.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(h->d_coef.p, h->coef.data(), h->coef.size() * sizeof(double), hipMemcpyHostToDevice));
+    h->tables.clear();  // onsite sums and move lists depend on ek, vk
+    h->solved = false;
+    return MOED_OK;
+}
+
+int build_table(moed_handle* h, int n, StrTable** out) {
+    auto it = h->tables.find(n);
+    if (it != h->tables.end()) {
+        *out = it->second.get();
+        return MOED_OK;
+    }
+    auto t = std::make_unique<StrTable>();
+    const int ns = h->nsite, norb = h->norb, nbath = h->nbath;
+    t->n = n;
+    t->rank.assign((size_t)1 << ns, -1);
+    for (uint32_t s = 0; s < (1u << ns); ++s)
+        if (__builtin_popcount(s) == n) {
+            t->rank[s] = (int32_t)t->str.size();
+            t->str.push_back(s);
+        }
+    t->count = (int)t->str.size();
+    const int C = t->count;
+    std::vector<std::vector<uint32_t>> moves(C);
+    std::vector<double> onsite(C);
+    std::vector<int32_t> imp((size_t)norb * norb * C, -1);
+    for (int r = 0; r < C; ++r) {
+        const uint32_t s = t->str[r];
+        double S = 0.0;
+        for (int site = 0; site < ns; ++site)
+            if ((s >> site) & 1u) S = S + h->ek[site];
+        onsite[r] = S;
+        auto& mv = moves[r];
+        for (int o = 0; o < norb; ++o)
+            for (int b = 0; b < nbath; ++b) {
+                const int ci = o + norb * b;
+                if (h->coef[ci] == 0.0) continue;
+                const int sb = norb + b;
+                const uint32_t ho = (s >> o) & 1u, hb = (s >> sb) & 1u;
+                if (ho == hb) continue;
+                const uint32_t tgt = s ^ (1u << o) ^ (1u << sb);
+                mv.push_back((uint32_t)t->rank[tgt] | ((uint32_t)ci << 16) | ((uint32_t)move_parity(s, o, sb) << 31));
+            }
+        if (h->has_tij)
+            for (int i = 0; i < norb; ++i)
+                for (int j = 0; j < norb; ++j) {
+                    const int ci = norb * nbath + i + norb * j;
+                    if (i == j || h->coef[ci] == 0.0) continue;
+                    if (((s >> j) & 1u) && !((s >> i) & 1u)) {
+                        const uint32_t tgt = s ^ (1u << i) ^ (1u << j);
+                        mv.push_back((uint32_t)t->rank[tgt] | ((uint32_t)ci << 16) |
+                                     ((uint32_t)move_parity(s, i, j) << 31));
+                    }
+                }
+        if ((int)mv.size() > kMaxHop) return fail(h, MOED_EARG, "too many one-body moves per string");
+        t->maxhop = std::max(t->maxhop, (int)mv.size());
+        for (int a = 0; a < norb; ++a)
+            for (int b = 0; b < norb; ++b) {
+                if (a == b) continue;
+                if (((s >> b) & 1u) && !((s >> a) & 1u)) {
+                    const uint32_t tgt = s ^ (1u << a) ^ (1u << b);
+                    imp[(size_t)(a * norb + b) * C + r] = t->rank[tgt] | (move_parity(s, a, b) << 16);
+                }
+            }
+    }
+    const int mh = std::max(t->maxhop, 1);
+    std::vector<uint32_t> hop((size_t)mh * C, 0u);
+    std::vector<uint8_t> nhop(C);
+    std::vector<uint16_t> str16(C);
+    for (int r = 0; r < C; ++r) {
+        nhop[r] = (uint8_t)moves[r].size();
+        str16[r] = (uint16_t)t->str[r];
+        for (size_t k = 0; k < moves[r].size(); ++k) hop[k * C + r] = moves[r][k];
+    }
+    HIPCHK(t->d_str.ensure(C * sizeof(uint16_t)));
+    HIPCHK(t->d_onsite.ensure(C * sizeof(double)));
+    HIPCHK(t->d_nhop.ensure(C));
+    HIPCHK(t->d_hop.ensure(hop.size() * sizeof(uint32_t)));
+    HIPCHK(t->d_imp.ensure(std::max<size_t>(imp.size(), 1) * sizeof(int32_t)));
+    HIPCHK(hipMemcpy(t->d_str.p, str16.data(), C * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_onsite.p, onsite.data(), C * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_nhop.p, nhop.data(), C, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_hop.p, hop.data(), hop.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!imp.empty()) HIPCHK(hipMemcpy(t->d_imp.p, imp.data(), imp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    *out = t.get();
+    h->tables[n] = std::move(t);
+    return MOED_OK;
+}
+
+struct Sector {
+    int nu, nd;
+    int64_t D;
+    StrTable* up;
+    StrTable* dn;
+};
+
+int get_sector(moed_handle* h, int nu, int nd, Sector* s) {
+    if (nu < 0 || nd < 0 || nu > h->nsite || nd > h->nsite)
+        return fail(h, MOED_ESECTOR, "sector (" + std::to_string(nu) + "," + std::to_string(nd) + ") out of range");
+    int rc = build_table(h, nu, &s->up);
+    if (rc) return rc;
+    rc = build_table(h, nd, &s->dn);
+    if (rc) return rc;
+    s->nu = nu;
+    s->nd = nd;
+    s->D = (int64_t)s->up->count * s->dn->count;
+    return MOED_OK;
+}
+
+void hv_shape(moed_handle* h, const Sector& s, int* threads, int* rows) {
+    int t = h->hv_threads, r = h->hv_rows;
+    if (t <= 0) t = 256;
+    if (r <= 0) r = 1;
+    while (r > 1 && (size_t)r * s.up->count * sizeof(double) > 64 * 1024) r >>= 1;
+    *threads = t;
+    *rows = r;
+}
+
+int hv_groups(moed_handle* h, const Sector& s) {
+    int t, r;
+    hv_shape(h, s, &t, &r);
+    return (s.dn->count + r - 1) / r;
+}
+
+int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* dot_part, const double* yscale,
+           const int32_t* stop, hipStream_t st) {
+    HvArgs a{};
+    a.x = x;
+    a.y = y;
+    a.up = s.up->dev();
+    a.dn = s.dn->dev();
+    a.ek = h->d_ek.as<double>();
+    a.coef = h->d_coef.as<double>();
+    a.ncoef = (int)h->coef.size();
+    a.nsite = h->nsite;
+    a.U = h->U;
+    a.UmJ3 = h->UmJ3;
+    a.Um2J = h->Um2J;
+    a.J = h->J;
+    a.dot_part = dot_part;
+    a.yscale = yscale;
+    a.stop = stop;
+    int t, r;
+    hv_shape(h, s, &t, &r);
+    a.rows_per_block = r;
+    HIPCHK(launch_hv(a, h->norb, t, st));
+    return MOED_OK;
+}
+
+// target sector of c+ (pm=1) / c (pm=2) on spin ispin
+void target_sector(int pm, int ispin, int nu, int nd, int* nuo, int* ndo) {
+    const int d = pm == 1 ? 1 : -1;
+    *nuo = nu + (ispin == 1 ? d : 0);
+    *ndo = nd + (ispin == 2 ? d : 0);
+}
+
+// y = c(+)_{iorb,ispin} x: sign of the operator evaluated on the target state, as apply_c does
+// (ed_operators.F90:44-58 calls destruction_op/creation_op on basis_out states).
+int apply_c_dev(moed_handle* h, int pm, int iorb, int ispin, const Sector& in, const Sector& out, const double* x,
+                double* y, hipStream_t st) {
+    const int o = iorb - 1;
+    const StrTable* tin = ispin == 1 ? in.up : in.dn;
+    const StrTable* tout = ispin == 1 ? out.up : out.dn;
+    std::vector<int32_t> map(tout->count, -1);
+    for (int r = 0; r < tout->count; ++r) {
+        const uint32_t s = tout->str[r];
+        const bool occ = (s >> o) & 1u;
+        if ((pm == 1) != occ) continue;  // c+ |src> = |tgt> needs bit o set in tgt; c needs it clear
+        const uint32_t src = s ^ (1u << o);
+        int par = __builtin_popcount(s & ((1u << o) - 1u)) & 1;
+        if (ispin == 2) par ^= (in.nu & 1);  // all up bits lie below the down bits (ed_operators.F90:65)
+        map[r] = tin->rank[src] | (par << 30);
+    }
+    DevBuf& mb = h->ws[WS_MISC2];
+    HIPCHK(mb.ensure(map.size() * sizeof(int32_t)));
+    HIPCHK(hipMemcpyAsync(mb.p, map.data(), map.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    ApplyCArgs a{x, y, mb.as<int32_t>(), out.up->count, out.dn->count, in.up->count, ispin == 1 ? 0 : 1};
+    HIPCHK(launch_apply_c(a, st));
+    HIPCHK(hipStreamSynchronize(st));  // map buffer is reused by the next call
+    return MOED_OK;
+}
+
+int dot_dev(moed_handle* h, const double* a, const double* b, int64_t n, double* out_host, hipStream_t st) {
+    DevBuf& part = h->ws[WS_PART];
+    HIPCHK(part.ensure((kRedBlocks + 8) * sizeof(double)));
+    double* p = part.as<double>();
+    HIPCHK(launch_dot_partial(a, b, n, p, st));
+    HIPCHK(launch_reduce_final(p, kRedBlocks, p + kRedBlocks, 0, 0, nullptr, st));
+    HIPCHK(hipMemcpyAsync(out_host, p + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+const double kBreakdown = (double)1.e-12f;  // ed_lanczos.F90:62, a REAL*4 literal
+
+// lanczos_iteration (ed_lanczos.F90:14-68) with the vectors kept on the device.
+int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep, int* nsc, double* a, double* b) {
+    if (nstep < 1) return fail(h, MOED_EARG, "nstep must be >= 1");
+    hipStream_t st = h->stream;
+    const int64_t D = s.D;
+    const int ngrp = hv_groups(h, s);
+    DevBuf& V = h->ws[WS_V];
+    DevBuf& W = h->ws[WS_W];
+    DevBuf& part = h->ws[WS_PART];
+    DevBuf& scal = h->ws[WS_SCAL];
+    HIPCHK(V.ensure((size_t)3 * D * sizeof(double)));
+    HIPCHK(W.ensure((size_t)D * sizeof(double)));
+    HIPCHK(part.ensure((size_t)(std::max(kRedBlocks, ngrp) + 8) * sizeof(double)));
+    HIPCHK(scal.ensure((size_t)(2 * (nstep + 1) + 2) * sizeof(double)));
+    double* u[3] = {V.as<double>(), V.as<double>() + D, V.as<double>() + 2 * D};
+    double* w = W.as<double>();
+    double* p = part.as<double>();
+    LzScal sc{scal.as<double>(), scal.as<double>() + (nstep + 1),
+              reinterpret_cast<int32_t*>(scal.as<double>() + 2 * (nstep + 1)),
+              reinterpret_cast<int32_t*>(scal.as<double>() + 2 * (nstep + 1)) + 1};
+    HIPCHK(launch_fill(0.0, 2 * (nstep + 1), scal.as<double>(), st));
+    const int32_t init[2] = {0, nstep};
+    HIPCHK(hipMemcpyAsync(sc.stop, init, sizeof(init), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(u[0], u0, D * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // B[0] = |u_0|^2 ; w = H u_0 ; A[0] = <u_0, w> / B[0]
+    HIPCHK(launch_dot_partial(u[0], u[0], D, p, st));
+    HIPCHK(launch_lz_finish_b(0, p, kRedBlocks, sc, st));
+    int rc = hv_dev(h, s, u[0], w, p, nullptr, sc.stop, st);
+    if (rc) return rc;
+    HIPCHK(launch_lz_finish_a(0, p, ngrp, sc, kBreakdown, st));
+    for (int j = 1; j <= nstep; ++j) {
+        HIPCHK(launch_lz_update(j, w, u[(j - 1) % 3], u[(j + 1) % 3], u[j % 3], D, sc, p, st));
+        HIPCHK(launch_lz_finish_b(j, p, kRedBlocks, sc, st));
+        rc = hv_dev(h, s, u[j % 3], w, p, nullptr, sc.stop, st);
+        if (rc) return rc;
+        HIPCHK(launch_lz_finish_a(j, p, ngrp, sc, kBreakdown, st));
+    }
+    std::vector<double> hs(2 * (nstep + 1) + 2);
+    HIPCHK(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j <= nstep; ++j) {
+        a[j] = hs[j];
+        b[j] = hs[nstep + 1 + j];
+    }
+    b[0] = 0.0;  // ed_lanczos.F90:40
+    int32_t flags[2];
+    std::memcpy(flags, &hs[2 * (nstep + 1)], sizeof(flags));
+    *nsc = flags[1];
+    // entries past a breakdown are zero, as the reference leaves them (ed_lanczos.F90:27-28)
+    for (int j = *nsc + 1; j <= nstep; ++j) a[j] = b[j] = 0.0;
+    return MOED_OK;
+}
+
+// ----------------------------------------------------------------- eigensolver (replaces PARPACK diag)
+
+int eig_dense(moed_handle* h, const Sector& s, int nev, SectorEig& E) {
+    const int D = (int)s.D;
+    hipStream_t st = h->stream;
+    DevBuf& X = h->ws[WS_X];
+    DevBuf& Y = h->ws[WS_Y];
+    HIPCHK(X.ensure((size_t)D * D * sizeof(double)));
+    HIPCHK(Y.ensure((size_t)D * D * sizeof(double)));
+    std::vector<double> I((size_t)D * D, 0.0), Hm((size_t)D * D);
+    for (int j = 0; j < D; ++j) I[(size_t)j * D + j] = 1.0;
+    HIPCHK(hipMemcpyAsync(X.p, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    for (int j = 0; j < D; ++j) {
+        int rc = hv_dev(h, s, X.as<double>() + (size_t)j * D, Y.as<double>() + (size_t)j * D, nullptr, nullptr,
+                        nullptr, st);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(Hm.data(), Y.p, Hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // column j of H is Hm[j*D : (j+1)*D]; H is symmetric, use it as row-major
+    std::vector<double> w, Vm;
+    jacobi_eig(Hm, D, w, Vm);
+    const int nf = std::min(nev, D);
+    E.nfound = nf;
+    E.ev.assign(nev, DBL_MAX);
+    std::vector<double> vecs((size_t)nf * D);
+    for (int i = 0; i < nf; ++i) {
+        E.ev[i] = w[i];
+        for (int r = 0; r < D; ++r) vecs[(size_t)i * D + r] = Vm[(size_t)r * D + i];
+    }
+    HIPCHK(E.vec.ensure(vecs.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(E.vec.p, vecs.data(), vecs.size() * sizeof(double), hipMemcpyHostToDevice));
+    return MOED_OK;
+}
+
+// Lanczos with full reorthogonalisation (classical Gram-Schmidt with a DGKS second pass),
+// stopping when the lowest nev Ritz pairs have relative residual |beta_m z_{m,i}| <= tol |theta_i|
+// or the Krylov space is exhausted; Ritz vectors formed as V S on the device.
+int eig_lanczos(moed_handle* h, const Sector& s, int nev, uint64_t seed, SectorEig& E) {
+    hipStream_t st = h->stream;
+    const int64_t D = s.D;
+    const int mmax = (int)std::min<int64_t>(h->gs_maxvec, D);
+    DevBuf& V = h->ws[WS_V];
+    DevBuf& W = h->ws[WS_W];
+    DevBuf& part = h->ws[WS_PART];
+    DevBuf& sc = h->ws[WS_SCAL];
+    const int nblk = multidot_blocks(D);
+    HIPCHK(V.ensure((size_t)(mmax + 1) * D * sizeof(double)));
+    HIPCHK(W.ensure((size_t)D * sizeof(double)));
+    HIPCHK(part.ensure((size_t)(mmax + 2) * nblk * sizeof(double) + 64 * sizeof(double)));
+    HIPCHK(sc.ensure((size_t)(4 * (mmax + 2)) * sizeof(double)));
+    double* Vp = V.as<double>();
+    double* w = W.as<double>();
+    double* pp = part.as<double>();
+    double* d_h = sc.as<double>();            // [mmax+2]
+    double* d_c = d_h + (mmax + 2);           // [mmax+2]
+    double* d_nrm = d_c + (mmax + 2);         // [mmax+2]
+    double* d_misc = d_nrm + (mmax + 2);      // scalars
+    std::vector<double> nrm(mmax + 2), alpha, beta, hh(mmax + 2), cc(mmax + 2);
+    HIPCHK(launch_fill_start(seed, D, Vp, st));
+    double n0;
+    int rc = dot_dev(h, Vp, Vp, D, &n0, st);
+    if (rc) return rc;
+    nrm[0] = std::sqrt(n0);
+    HIPCHK(hipMemcpyAsync(d_nrm, nrm.data(), sizeof(double), hipMemcpyHostToDevice, st));
+    const double eta2 = 0.5;  // DGKS: second pass when |w'|^2 < |w|^2 / 2
+    int m = 0;
+    bool done = false;
+    std::vector<double> ritz_z;  // last row of Z at the last check
+    while (!done) {
+        const int k = m;
+        double* uk = Vp + (size_t)k * D;
+        rc = hv_dev(h, s, uk, w, nullptr, d_nrm + k, nullptr, st);  // w = H v_k
+        if (rc) return rc;
+        double alpha_k = 0.0, wnorm2 = 0.0, wp2 = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            HIPCHK(launch_multidot(Vp, D, k + 1, pass == 0 ? w : Vp + (size_t)(k + 1) * D, D, pp, nblk, st));
+            HIPCHK(launch_multidot_final(pp, k + 1, nblk, d_h, st));
+            HIPCHK(hipMemcpyAsync(hh.data(), d_h, (k + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int i = 0; i <= k; ++i) cc[i] = hh[i] / (nrm[i] * nrm[i]);
+            alpha_k += hh[k] / nrm[k];
+            if (pass == 0) wnorm2 = hh[k + 1];
+            HIPCHK(hipMemcpyAsync(d_c, cc.data(), (k + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(launch_project_out(Vp, D, k + 1, d_c, pass == 0 ? w : Vp + (size_t)(k + 1) * D,
+                                      Vp + (size_t)(k + 1) * D, D, pp, nblk, st));
+            HIPCHK(launch_reduce_final(pp, nblk, d_misc, 0, 0, nullptr, st));
+            HIPCHK(hipMemcpyAsync(&wp2, d_misc, sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (wp2 >= eta2 * wnorm2) break;
+            wnorm2 = wp2;
+        }
+        alpha.push_back(alpha_k);
+        const double bk = std::sqrt(wp2);
+        m = k + 1;
+        // convergence check on T_m
+        double tnorm = 0.0;
+        for (int i = 0; i < m; ++i) tnorm = std::max(tnorm, std::fabs(alpha[i]) + (i < (int)beta.size() ? beta[i] : 0.0));
+        const bool exhausted = bk <= 1e-14 * std::max(tnorm, 1e-300) || m >= mmax;
+        bool check = exhausted || (m >= nev && (m % 5 == 0 || m < 20));
+        if (check) {
+            std::vector<double> d(alpha), e(beta);
+            std::vector<double> Z((size_t)m, 0.0);
+            Z[m - 1] = 1.0;  // the last row of the identity
+            if (!tridiag_ql(d, e, 1, &Z)) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
+            const int nc = std::min(nev, m);
+            bool conv = m >= nev || exhausted;
+            for (int i = 0; i < nc && conv; ++i)
+                if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) conv = false;
+            if (conv || exhausted) done = true;
+        }
+        if (!done) {
+            beta.push_back(bk);
+            nrm[m] = bk;
+            HIPCHK(hipMemcpyAsync(d_nrm + m, &nrm[m], sizeof(double), hipMemcpyHostToDevice, st));
+        }
+    }
+    // Ritz vectors from the full eigenvector matrix of T_m
+    std::vector<double> d(alpha), e(beta);
+    std::vector<double> Z((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+    if (!tridiag_ql(d, e, m, &Z)) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
+    const int nf = std::min(nev, m);
+    E.nfound = nf;
+    E.ev.assign(nev, DBL_MAX);
+    for (int i = 0; i < nf; ++i) E.ev[i] = d[i];
+    std::vector<double> S((size_t)m * nf);
+    for (int i = 0; i < nf; ++i)
+        for (int k = 0; k < m; ++k) S[k + (size_t)m * i] = Z[(size_t)k * m + i] / nrm[k];
+    HIPCHK(E.vec.ensure((size_t)nf * D * sizeof(double)));
+    DevBuf& Sb = h->ws[WS_MISC];
+    HIPCHK(Sb.ensure(S.size() * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(Sb.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    for (int i0 = 0; i0 < nf; i0 += 16) {
+        const int cnt = std::min(16, nf - i0);
+        HIPCHK(launch_ritz(Vp, D, m, Sb.as<double>() + (size_t)m * i0, cnt, E.vec.as<double>() + (size_t)i0 * D, D, D,
+                           st));
+    }
+    // normalise
+    for (int i = 0; i < nf; ++i) {
+        double* y = E.vec.as<double>() + (size_t)i * D;
+        double n2;
+        rc = dot_dev(h, y, y, D, &n2, st);
+        if (rc) return rc;
+        HIPCHK(launch_scale_copy(y, 1.0 / std::sqrt(n2), D, y, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int eig_sector(moed_handle* h, int nu, int nd, int nev, SectorEig& E) {
+    Sector s;
+    int rc = get_sector(h, nu, nd, &s);
+    if (rc) return rc;
+    E.nu = nu;
+    E.nd = nd;
+    E.D = s.D;
+    if (s.D <= 64) return eig_dense(h, s, nev, E);
+    return eig_lanczos(h, s, nev, 0x5eed0000ull + (uint64_t)nu * 131 + nd, E);
+}
+
+// SORT of func.f90:301-353: heapsort returning a 1-based index, with the same sift order so that
+// ties come out in the same order.
+void heapsort_index(int n, const double* a, int* indx) {
+    const double* A = a - 1;
+    int* X = indx - 1;
+    if (n == 1) {
+        X[1] = 1;
+        return;
+    }
+    for (int j = 1; j <= n; ++j) X[j] = j;
+    int l = n / 2 + 1, ir = n;
+    while (true) {
+        int indxt;
+        double q;
+        if (l > 1) {
+            --l;
+            indxt = X[l];
+            q = A[indxt];
+        } else {
+            indxt = X[ir];
+            q = A[indxt];
+            X[ir] = X[1];
+            if (--ir == 1) {
+                X[1] = indxt;
+                return;
+            }
+        }
+        int i = l, j = l + l;
+        while (j <= ir) {
+            if (j < ir && A[X[j]] < A[X[j + 1]]) ++j;
+            if (q < A[X[j]]) {
+                X[i] = X[j];
+                i = j;
+                j += j;
+            } else {
+                j = ir + 1;
+            }
+        }
+        X[i] = indxt;
+    }
+}
+
+// boltzmann_factor (ed_solver.F90:193-219)
+void boltzmann(const std::vector<double>& e, double beta, std::vector<double>& p) {
+    const int n = (int)e.size();
+    p.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double Z = 0.0;
+        bool cut = false;
+        for (int k = 0; k < n; ++k) {
+            if (-beta * (e[k] - e[i]) > 10.0) {
+                cut = true;
+                break;
+            }
+            Z = Z + std::exp(-beta * (e[k] - e[i]));
+        }
+        p[i] = cut ? 0.0 : 1.0 / Z;
+    }
+}
+
+int green_diag(moed_handle* h, const Sector& s, const double* psi, double E, double weight, int iorb, int ispin,
+               const double* d_omega, int nw, double* d_G, GreenRecord& rec) {
+    hipStream_t st = h->stream;
+    const int compat = (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1;
+    const int nstep = h->nstep;
+    double normsq = 0.0;
+    for (int pm = 1; pm <= 2; ++pm) {
+        int nuo, ndo;
+        target_sector(pm, ispin, s.nu, s.nd, &nuo, &ndo);
+        std::vector<double> a(nstep + 1, 0.0), b(nstep + 1, 0.0);
+        int nsc = 0;
+        bool have = nuo >= 0 && ndo >= 0 && nuo <= h->nsite && ndo <= h->nsite;
+        double n2 = 0.0;
+        if (have) {
+            Sector so;
+            int rc = get_sector(h, nuo, ndo, &so);
+            if (rc) return rc;
+            DevBuf& X = h->ws[WS_X];
+            HIPCHK(X.ensure((size_t)so.D * sizeof(double)));
+            rc = apply_c_dev(h, pm, iorb, ispin, s, so, psi, X.as<double>(), st);
+            if (rc) return rc;
+            rc = dot_dev(h, X.as<double>(), X.as<double>(), so.D, &n2, st);
+            if (rc) return rc;
+            if (n2 > 0.0) {
+                rc = lanczos_ref_dev(h, so, X.as<double>(), nstep, &nsc, a.data(), b.data());
+                if (rc) return rc;
+            } else {
+                have = false;  // c(+)|psi> = 0: no contribution
+            }
+        }
+        if (pm == 1) normsq = n2;
+        b[0] = pm == 1 ? normsq : 1.0 - normsq;  // ed_green.F90:158, :174
+        if (have) {
+            DevBuf& AB = h->ws[WS_MISC];
+            HIPCHK(AB.ensure((size_t)2 * (nstep + 1) * sizeof(double)));
+            HIPCHK(hipMemcpyAsync(AB.p, a.data(), (nstep + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(AB.as<double>() + nstep + 1, b.data(), (nstep + 1) * sizeof(double),
+                                  hipMemcpyHostToDevice, st));
+            CfArgs cf{AB.as<double>(), AB.as<double>() + nstep + 1, nsc, pm == 1 ? 1 : -1, E, weight, d_omega, nw,
+                      compat, d_G + 2 * (iorb - 1), h->norb};
+            HIPCHK(launch_cf(cf, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        if (pm == 1) {
+            rec.nsp = nsc;
+            rec.ap = a;
+            rec.bp = b;
+        } else {
+            rec.nsn = nsc;
+            rec.an = a;
+            rec.bn = b;
+        }
+    }
+    return MOED_OK;
+}
+
+}  // namespace
+
+static FitCtx fit_ctx(moed_handle* h) {
+    return FitCtx{h->stream, h->norb, h->nbath, (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1, &h->err};
+}
+
+int fit_delta_host(moed_handle* h, int nx, const double* x, int nw, const double* omega, double* Dev) {
+    (void)hipSetDevice(h->device);
+    return fit_delta(fit_ctx(h), nx, x, nw, omega, Dev);
+}
+
+int fit_eval_host(moed_handle* h, int nfit, int nx, const double* x, int nw, const double* omega, const double* Dev,
+                  double* f, double* g) {
+    (void)hipSetDevice(h->device);
+    return fit_eval(fit_ctx(h), nfit, nx, x, nw, omega, Dev, f, g);
+}
+
+int fit_minimize_host(moed_handle* h, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
+                      double ftol, double* fret, int* iter, int* status) {
+    (void)hipSetDevice(h->device);
+    return fit_minimize(fit_ctx(h), nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
+}
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int moed_create(const moed_params* p, moed_handle** out) {
+    if (!p || !out) return MOED_EARG;
+    *out = nullptr;
+    if (p->norb < 1 || p->norb > kMaxOrb || p->nsite < p->norb || p->nsite > kMaxSite || !p->ek || !p->vk)
+        return MOED_EARG;
+    if (p->nstep < 0 || p->nev < 0 || p->nw < 0 || p->nsector < 0) return MOED_EARG;
+    auto* h = new moed_handle();
+    h->device = p->device;
+    h->norb = p->norb;
+    h->nsite = p->nsite;
+    h->nbath = p->nsite - p->norb;
+    h->U = p->U;
+    h->J = p->J;
+    h->beta = p->beta;
+    h->nstep = p->nstep;
+    h->nev = p->nev;
+    h->nw = p->nw;
+    h->flags = p->flags;
+    h->gs_tol = p->gs_tol > 0 ? p->gs_tol : 1e-12;
+    h->gs_maxvec = p->gs_maxvec > 0 ? p->gs_maxvec : 400;
+    if (p->tij) {
+        h->has_tij = true;
+        h->tij.assign(p->tij, p->tij + p->norb * p->norb);
+    }
+    for (int i = 0; i < p->nsector; ++i) {
+        h->nelec.push_back(p->nelec[i]);
+        h->nup.push_back(p->nup[i]);
+    }
+    interaction_constants(h->U, h->J, &h->UmJ3, &h->Um2J);
+    hipError_t e = hipSetDevice(h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
+    }
+    int rc = set_model(h, p->ek, p->vk);
+    if (rc) {
+        moed_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return MOED_OK;
+}
+
+int moed_destroy(moed_handle* h) {
+    if (!h) return MOED_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->tables.clear();
+    h->eig.clear();
+    for (auto& w : h->ws) w.release();
+    h->d_ek.release();
+    h->d_coef.release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return MOED_OK;
+}
+
+int moed_last_error(const moed_handle* h, char* buf, int32_t len) {
+    if (!h || !buf || len <= 0) return MOED_EARG;
+    std::snprintf(buf, (size_t)len, "%s", h->err.c_str());
+    return MOED_OK;
+}
+
+int moed_set_bath(moed_handle* h, const double* ek, const double* vk) {
+    if (!h || !ek || !vk) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    return set_model(h, ek, vk);
+}
+
+int moed_sector_dim(moed_handle* h, int32_t nu, int32_t nd, int64_t* dim) {
+    if (!h || !dim) return MOED_EARG;
+    if (nu < 0 || nd < 0 || nu > h->nsite || nd > h->nsite) return fail(h, MOED_ESECTOR, "sector out of range");
+    *dim = binom(h->nsite, nu) * binom(h->nsite, nd);
+    return MOED_OK;
+}
+
+int moed_basis(moed_handle* h, int32_t nu, int32_t nd, int32_t* up_states, int32_t* dn_states) {
+    if (!h) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    Sector s;
+    int rc = get_sector(h, nu, nd, &s);
+    if (rc) return rc;
+    if (up_states)
+        for (int i = 0; i < s.up->count; ++i) up_states[i] = (int32_t)s.up->str[i];
+    if (dn_states)
+        for (int i = 0; i < s.dn->count; ++i) dn_states[i] = (int32_t)s.dn->str[i];
+    return MOED_OK;
+}
+
+int moed_hv_device(moed_handle* h, int32_t nu, int32_t nd, const double* x, double* y, void* stream) {
+    if (!h || !x || !y) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    Sector s;
+    int rc = get_sector(h, nu, nd, &s);
+    if (rc) return rc;
+    return hv_dev(h, s, x, y, nullptr, nullptr, nullptr, stream ? (hipStream_t)stream : h->stream);
+}
+
+int moed_hv(moed_handle* h, int32_t nu, int32_t nd, const double* x, double* y) {
+    if (!h || !x || !y) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    Sector s;
+    int rc = get_sector(h, nu, nd, &s);
+    if (rc) return rc;
+    hipStream_t st = h->stream;
+    DevBuf& X = h->ws[WS_X];
+    DevBuf& Y = h->ws[WS_Y];
+    HIPCHK(X.ensure(s.D * sizeof(double)));
+    HIPCHK(Y.ensure(s.D * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(X.p, x, s.D * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = hv_dev(h, s, X.as<double>(), Y.as<double>(), nullptr, nullptr, nullptr, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(y, Y.p, s.D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_apply_c(moed_handle* h, int32_t pm, int32_t iorb, int32_t ispin, int32_t nu, int32_t nd, const double* x,
+                 double* y) {
+    if (!h || !x || !y || (pm != 1 && pm != 2) || iorb < 1 || iorb > h->norb || (ispin != 1 && ispin != 2))
+        return h ? fail(h, MOED_EARG, "apply_c: bad arguments") : MOED_EARG;
+    (void)hipSetDevice(h->device);
+    Sector si, so;
+    int rc = get_sector(h, nu, nd, &si);
+    if (rc) return rc;
+    int nuo, ndo;
+    target_sector(pm, ispin, nu, nd, &nuo, &ndo);
+    rc = get_sector(h, nuo, ndo, &so);
+    if (rc) return rc;
+    hipStream_t st = h->stream;
+    DevBuf& X = h->ws[WS_X];
+    DevBuf& Y = h->ws[WS_Y];
+    HIPCHK(X.ensure(si.D * sizeof(double)));
+    HIPCHK(Y.ensure(so.D * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(X.p, x, si.D * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = apply_c_dev(h, pm, iorb, ispin, si, so, X.as<double>(), Y.as<double>(), st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(y, Y.p, so.D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_lanczos(moed_handle* h, int32_t nu, int32_t nd, const double* vec, int32_t nstep, int32_t* nsc, double* a,
+                 double* b) {
+    if (!h || !vec || !nsc || !a || !b || nstep < 1) return h ? fail(h, MOED_EARG, "lanczos: bad arguments") : MOED_EARG;
+    (void)hipSetDevice(h->device);
+    Sector s;
+    int rc = get_sector(h, nu, nd, &s);
+    if (rc) return rc;
+    DevBuf& X = h->ws[WS_X];
+    HIPCHK(X.ensure(s.D * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(X.p, vec, s.D * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    double n2;
+    rc = dot_dev(h, X.as<double>(), X.as<double>(), s.D, &n2, h->stream);
+    if (rc) return rc;
+    if (!(n2 > 0.0)) return fail(h, MOED_EZERO, "lanczos: zero start vector");
+    int c = 0;
+    rc = lanczos_ref_dev(h, s, X.as<double>(), nstep, &c, a, b);
+    *nsc = c;
+    return rc;
+}
+
+int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int32_t* sector, int32_t* level) {
+    if (!h || !nev_calc) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    const int nsec = (int)h->nelec.size(), nev = h->nev;
+    if (nsec < 1 || nev < 1) return fail(h, MOED_EARG, "solve: need nsector >= 1 and nev >= 1");
+    h->eig.clear();
+    h->solved = false;
+    std::vector<double> all((size_t)nsec * nev);
+    for (int is = 0; is < nsec; ++is) {
+        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+        auto E = std::make_unique<SectorEig>();
+        int rc = eig_sector(h, nu, nd, nev, *E);
+        if (rc) return rc;
+        for (int i = 0; i < nev; ++i) all[(size_t)is * nev + i] = E->ev[i];
+        h->eig.push_back(std::move(E));
+    }
+    // ed_solver.F90:65-90
+    std::vector<int> ind(all.size());
+    heapsort_index((int)all.size(), all.data(), ind.data());
+    std::vector<double> pev;
+    boltzmann(all, h->beta, pev);
+    int nc = 0;
+    for (int i = 0; i < nev && i < (int)all.size(); ++i)
+        if (pev[ind[i] - 1] > 0.001) ++nc;
+    h->kept_val.assign(nc, 0.0);
+    h->kept_prob.assign(nc, 0.0);
+    h->kept_sector.assign(nc, 0);
+    h->kept_level.assign(nc, 0);
+    double Z = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        const int k = ind[i] - 1;
+        h->kept_val[i] = all[k];
+        h->kept_prob[i] = pev[k];
+        h->kept_sector[i] = k / nev + 1;
+        h->kept_level[i] = k % nev + 1;
+        Z = Z + pev[k];
+    }
+    for (int i = 0; i < nc; ++i) h->kept_prob[i] = h->kept_prob[i] / Z;
+    *nev_calc = nc;
+    for (int i = 0; i < nc; ++i) {
+        if (val) val[i] = h->kept_val[i];
+        if (prob) prob[i] = h->kept_prob[i];
+        if (sector) sector[i] = h->kept_sector[i];
+        if (level) level[i] = h->kept_level[i];
+    }
+    h->solved = true;
+    return MOED_OK;
+}
+
+int moed_sector_eigenvalues(moed_handle* h, int32_t isector, double* ev) {
+    if (!h || !ev) return MOED_EARG;
+    if (!h->solved) return fail(h, MOED_ESTATE, "call moed_solve first");
+    if (isector < 1 || isector > (int)h->eig.size()) return fail(h, MOED_EARG, "sector index out of range");
+    const auto& E = *h->eig[isector - 1];
+    for (int i = 0; i < h->nev; ++i) ev[i] = E.ev[i];
+    return MOED_OK;
+}
+
+int moed_eigvec(moed_handle* h, int32_t isector, int32_t ilevel, double* vec) {
+    if (!h || !vec) return MOED_EARG;
+    if (!h->solved) return fail(h, MOED_ESTATE, "call moed_solve first");
+    if (isector < 1 || isector > (int)h->eig.size()) return fail(h, MOED_EARG, "sector index out of range");
+    const auto& E = *h->eig[isector - 1];
+    if (ilevel < 1 || ilevel > E.nfound) return fail(h, MOED_EARG, "level not available");
+    (void)hipSetDevice(h->device);
+    HIPCHK(hipMemcpy(vec, E.vec.as<double>() + (size_t)(ilevel - 1) * E.D, E.D * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return MOED_OK;
+}
+
+int moed_green(moed_handle* h, int32_t nw, const double* omega, double* Gr) {
+    if (!h || !omega || !Gr || nw < 1) return h ? fail(h, MOED_EARG, "green: bad arguments") : MOED_EARG;
+    if (!h->solved) return fail(h, MOED_ESTATE, "call moed_solve first");
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    const int norb = h->norb;
+    DevBuf om, G;
+    HIPCHK(om.ensure(nw * sizeof(double)));
+    HIPCHK(G.ensure((size_t)2 * norb * nw * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(om.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_fill(0.0, (int64_t)2 * norb * nw, G.as<double>(), st));
+    h->records.clear();
+    for (size_t iev = 0; iev < h->kept_val.size(); ++iev) {
+        const int is = h->kept_sector[iev] - 1, il = h->kept_level[iev] - 1;
+        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+        const bool even = nd == nu;  // ed_green.F90:87-94
+        const double factor = even ? 1.0 : 0.5;
+        const double weight = h->kept_prob[iev] * factor;
+        Sector s;
+        int rc = get_sector(h, nu, nd, &s);
+        if (rc) return rc;
+        const SectorEig& E = *h->eig[is];
+        if (il >= E.nfound) return fail(h, MOED_ESTATE, "kept state has no eigenvector");
+        const double* psi = E.vec.as<double>() + (size_t)il * E.D;
+        for (int ispin = 1; ispin <= (even ? 1 : 2); ++ispin)
+            for (int iorb = 1; iorb <= norb; ++iorb) {
+                GreenRecord rec{};
+                rec.iev = (int)iev + 1;
+                rec.iorb = iorb;
+                rec.ispin = ispin;
+                rec.E = h->kept_val[iev];
+                rec.p = h->kept_prob[iev];
+                rec.even = even;
+                rc = green_diag(h, s, psi, h->kept_val[iev], weight, iorb, ispin, om.as<double>(), nw, G.as<double>(),
+                                rec);
+                if (rc) return rc;
+                h->records.push_back(std::move(rec));
+            }
+    }
+    HIPCHK(hipMemcpyAsync(Gr, G.p, (size_t)2 * norb * nw * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_green_nrecords(moed_handle* h, int32_t* nrec) {
+    if (!h || !nrec) return MOED_EARG;
+    *nrec = (int32_t)h->records.size();
+    return MOED_OK;
+}
+
+int moed_green_record(moed_handle* h, int32_t irec, int32_t* iev, int32_t* iorb, int32_t* ispin, double* E, double* p,
+                      int32_t* even, int32_t* nsp, int32_t* nsn, double* ap, double* bp, double* an, double* bn) {
+    if (!h) return MOED_EARG;
+    if (irec < 0 || irec >= (int)h->records.size()) return fail(h, MOED_EARG, "record index out of range");
+    const GreenRecord& r = h->records[irec];
+    if (iev) *iev = r.iev;
+    if (iorb) *iorb = r.iorb;
+    if (ispin) *ispin = r.ispin;
+    if (E) *E = r.E;
+    if (p) *p = r.p;
+    if (even) *even = r.even;
+    if (nsp) *nsp = r.nsp;
+    if (nsn) *nsn = r.nsn;
+    const size_t n = h->nstep + 1;
+    if (ap) std::copy(r.ap.begin(), r.ap.begin() + n, ap);
+    if (bp) std::copy(r.bp.begin(), r.bp.begin() + n, bp);
+    if (an) std::copy(r.an.begin(), r.an.begin() + n, an);
+    if (bn) std::copy(r.bn.begin(), r.bn.begin() + n, bn);
+    return MOED_OK;
+}
+
+int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b, int32_t nw,
+                  const double* omega, double* g) {
+    if (!h || !a || !b || !omega || !g || nsc < 0 || nw < 1) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    DevBuf ab, om, G;
+    HIPCHK(ab.ensure((size_t)2 * (nsc + 1) * sizeof(double)));
+    HIPCHK(om.ensure(nw * sizeof(double)));
+    HIPCHK(G.ensure((size_t)2 * nw * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ab.p, a, (nsc + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ab.as<double>() + nsc + 1, b, (nsc + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(om.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_fill(0.0, 2 * nw, G.as<double>(), st));
+    const int compat = (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1;
+    CfArgs cf{ab.as<double>(), ab.as<double>() + nsc + 1, nsc, sign >= 0 ? 1 : -1, E, 1.0, om.as<double>(), nw, compat,
+              G.as<double>(), 1};
+    HIPCHK(launch_cf(cf, st));
+    HIPCHK(hipMemcpyAsync(g, G.p, (size_t)2 * nw * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_green_cf_adjoint(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
+                          int32_t nw, const double* omega, const double* gbar, double* ga, double* gb) {
+    if (!h || !a || !b || !omega || !gbar || !ga || !gb || nsc < 0 || nw < 1) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    const size_t n1 = nsc + 1;
+    DevBuf ab, om, gbr, scr, out;
+    HIPCHK(ab.ensure(2 * n1 * sizeof(double)));
+    HIPCHK(om.ensure(nw * sizeof(double)));
+    HIPCHK(gbr.ensure((size_t)2 * nw * sizeof(double)));
+    HIPCHK(scr.ensure(n1 * nw * 6 * sizeof(double)));
+    HIPCHK(out.ensure(2 * n1 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ab.p, a, n1 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ab.as<double>() + n1, b, n1 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(om.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(gbr.p, gbar, (size_t)2 * nw * sizeof(double), hipMemcpyHostToDevice, st));
+    const int compat = (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1;
+    CfArgs cf{ab.as<double>(), ab.as<double>() + n1, nsc, sign >= 0 ? 1 : -1, E, 1.0, om.as<double>(), nw, compat,
+              nullptr, 1};
+    HIPCHK(launch_cf_adjoint(cf, gbr.as<double>(), scr.as<double>(), out.as<double>(), out.as<double>() + n1, st));
+    HIPCHK(hipMemcpyAsync(ga, out.p, n1 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(gb, out.as<double>() + n1, n1 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_calc_dev(moed_handle* h, const double* ek, const double* vk, const double* ef, int32_t nw, const double* omega,
+                  double* Dev) {
+    if (!h || !ek || !vk || !ef || !omega || !Dev || nw < 1) return MOED_EARG;
+    // Delta(iw) of calc_dev is the fit's Delta at x = ev_to_x(ek, vk, ef) with a zero target
+    const int norb = h->norb, nbath = h->nbath, nx = nbath + nbath * norb + norb;
+    std::vector<double> x(nx);
+    for (int i = 0; i < nbath; ++i) x[i] = ek[norb + i];
+    for (int o = 0; o < norb; ++o)
+        for (int k = 0; k < nbath; ++k) x[nbath + o * nbath + k] = vk[o + (size_t)norb * k];
+    for (int o = 0; o < norb; ++o) x[nx - norb + o] = ef[o];
+    return fit_delta_host(h, nx, x.data(), nw, omega, Dev);
+}
+
+int moed_chi2_grad(moed_handle* h, int32_t nfit, int32_t nx, const double* x, int32_t nw, const double* omega,
+                   const double* Dev, double* f, double* g) {
+    if (!h || !x || !omega || !Dev || nfit < 1 || nw < 1) return MOED_EARG;
+    if (nx != h->nbath * (1 + h->norb) + h->norb) return fail(h, MOED_EARG, "nx != Nbath*(1+Norb)+Norb");
+    return fit_eval_host(h, nfit, nx, x, nw, omega, Dev, f, g);
+}
+
+int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t nw, const double* omega,
+                  const double* Dev, double ftol, double* fret, int32_t* iter, int32_t* status) {
+    if (!h || !x || !omega || !Dev || nfit < 1 || nw < 1) return MOED_EARG;
+    if (nx != h->nbath * (1 + h->norb) + h->norb) return fail(h, MOED_EARG, "nx != Nbath*(1+Norb)+Norb");
+    return fit_minimize_host(h, nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
+}
+
+int moed_omega_grid(int32_t nw, double beta, double* omega) {
+    if (nw < 0 || !omega || !(beta > 0)) return MOED_EARG;
+    const double pi = 4.0 * std::atan(1.0);
+    for (int i = 0; i < nw; ++i) omega[i] = (2.0 * (double)(float)i + 1) * pi / beta;
+    return MOED_OK;
+}
+
+int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x, void* stream) {
+    if (!h || !x || n < 0) return MOED_EARG;
+    (void)hipSetDevice(h->device);
+    HIPCHK(launch_fill_start(seed, n, x, stream ? (hipStream_t)stream : h->stream));
+    return MOED_OK;
+}
+
+int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t rows) {
+    if (!h) return MOED_EARG;
+    if (threads != 0 && threads != 256 && threads != 512 && threads != 1024)
+        return fail(h, MOED_EARG, "threads must be 0, 256, 512 or 1024");
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(h, MOED_EARG, "rows must be 0, 1, 2 or 4");
+    h->hv_threads = threads;
+    h->hv_rows = rows;
+    return MOED_OK;
+}
+
+void* moed_stream(moed_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// Internal declarations of libmoed: device table layout, kernel argument blocks and the
+// host-side launchers shared by moed_kernels.hip and moed_host.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/moed.h"
+
+namespace moed {
+
+constexpr int kMaxSite = 15;   // strings fit in uint16; reference state integer fits int32 (ed_config.f90:8)
+constexpr int kMaxOrb = 4;     // impurity orbitals
+constexpr int kMaxCoef = 128;  // distinct hopping coefficients (V_ob and t_ij)
+constexpr int kMaxHop = 72;    // one-body moves per string (<= Norb*Nbath + Norb*(Norb-1))
+constexpr int kXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// One popcount class of Nsite-bit strings ("up" or "down" strings of a sector), on the device.
+// Structure-of-arrays over the string rank r (0-based, ascending integer order = the reference's
+// generate_basis order, ed_basis.F90:87-105):
+//   str[r]                 the bit string
+//   onsite[r]              sum of ek over the occupied sites in ascending site order, starting from
+//                          0.0 (the first partial sum of multiply_H's diagonal, ed_hamiltonian.F90:112-123)
+//   nhop[r]                number of one-body moves with a nonzero coefficient (V_ob, t_ij)
+//   hop[k*count + r]       move k: target rank (bits 0-15) | coefficient index (bits 16-23) | sign (bit 31)
+//   imp[(a*norb+b)*count+r] impurity move b -> a (c+_a c_b), a != b: target rank | sign << 16, or -1
+struct StrDev {
+    const uint16_t* str;
+    const double* onsite;
+    const uint8_t* nhop;
+    const uint32_t* hop;
+    const int32_t* imp;
+    int32_t count;
+    int32_t maxhop;
+};
+
+struct HvArgs {
+    const double* x;
+    double* y;
+    StrDev up, dn;
+    const double* ek;    // [nsite]
+    const double* coef;  // [ncoef]
+    int32_t ncoef;
+    int32_t nsite;
+    double U, UmJ3, Um2J, J;  // U, U-3J, U-2J, J exactly as the reference forms them (ed_operators.F90:156-162)
+    double* dot_part;         // optional: per-row partial of sum_i x_i y_i (nullptr = off)
+    const double* yscale;     // optional device scalar: y *= 1/yscale[0] (nullptr = off)
+    const int32_t* stop;      // optional device flag: kernel is a no-op when *stop != 0
+    int32_t rows_per_xcd;
+    int32_t rows_per_block;
+};
+
+// apply_c: per target string, the source rank and sign of the elementary operator
+struct ApplyCArgs {
+    const double* x;
+    double* y;
+    const int32_t* map;  // [n_target strings of the changed spin]: src rank | (sign bit 30), or -1
+    int32_t nU_out, nD_out, nU_in;
+    int32_t spin;  // 0: up strings change (column map), 1: down strings change (row map)
+};
+
+// launchers (moed_kernels.hip)
+hipError_t launch_hv(const HvArgs& a, int norb, int threads, hipStream_t s);
+hipError_t launch_apply_c(const ApplyCArgs& a, hipStream_t s);
+hipError_t launch_fill_start(uint64_t seed, int64_t n, double* x, hipStream_t s);
+hipError_t launch_fill(double v, int64_t n, double* x, hipStream_t s);
+hipError_t launch_scale_copy(const double* x, double alpha, int64_t n, double* y, hipStream_t s);
+
+// deterministic reductions: partial sums per block then one final block in fixed order
+constexpr int kRedBlocks = 1024;
+hipError_t launch_dot_partial(const double* a, const double* b, int64_t n, double* part, hipStream_t s);
+// out[slot] = op(sum(part[0:npart]))  op: 0 plain, 1 divide by scal[div_slot], 2 sqrt
+hipError_t launch_reduce_final(const double* part, int npart, double* out, int slot, int op, const double* divp,
+                               hipStream_t s);
+
+// Lanczos (reference recurrence, ed_lanczos.F90:37-68) on unnormalised vectors u_j:
+//   u_j = (w - A[j-1] u_{j-1}) / sqrt(B[j-1]) - sqrt(B[j-1]/B[j-2]) u_{j-2}
+// scal layout: A[0..nstep], B[0..nstep] (B[0] = |u_0|^2 here), stop flag and nstep_calc as int at the end
+struct LzScal {
+    double* A;
+    double* B;
+    int32_t* stop;
+    int32_t* nsc;
+};
+hipError_t launch_lz_update(int j, const double* w, const double* u1, const double* u0, double* out, int64_t n,
+                            LzScal sc, double* part, hipStream_t s);
+// B[j] = sum(part); for j >= 2 nothing else.  A[j] = sum(part)/B[j]; if j>=2 and B[j] < thr: stop=1, nsc=j.
+hipError_t launch_lz_finish_b(int j, const double* part, int npart, LzScal sc, hipStream_t s);
+hipError_t launch_lz_finish_a(int j, const double* part, int npart, LzScal sc, double thr, hipStream_t s);
+
+// full-reorthogonalisation Lanczos for eigenpairs: V holds unnormalised u_0..u_k (column stride ld)
+// with norms nrm[0..k]; pass 1: h[i] = <u_i, w>, and |w|^2; pass 2: w' = w - sum_i c_i u_i -> V[:,k+1], |w'|^2
+hipError_t launch_multidot(const double* V, int64_t ld, int k1, const double* w, int64_t n, double* part,
+                           int nblk, hipStream_t s);
+hipError_t launch_multidot_final(const double* part, int k1, int nblk, double* h, hipStream_t s);
+hipError_t launch_project_out(const double* V, int64_t ld, int k1, const double* c, const double* w, double* out,
+                              int64_t n, double* part, int nblk, hipStream_t s);
+int multidot_blocks(int64_t n);
+// Ritz vectors: Y[:, i] = sum_k S[k, i] V[:, k] for i < nev (S column-major m x nev, already divided by nrm)
+hipError_t launch_ritz(const double* V, int64_t ld, int m, const double* S, int nev, double* Y, int64_t ldy,
+                       int64_t n, hipStream_t s);
+
+// continued fraction over all frequencies, accumulating w*g into G (interleaved complex, stride)
+struct CfArgs {
+    const double* a;
+    const double* b;
+    int32_t n;      // nstep_calc
+    int32_t sign;   // +1 particle, -1 hole
+    double E;
+    double weight;  // pev*factor
+    const double* omega;
+    int32_t nw;
+    int32_t compat;
+    double* G;      // interleaved complex, element i at G[2*i*stride]
+    int32_t stride;
+};
+hipError_t launch_cf(const CfArgs& a, hipStream_t s);
+hipError_t launch_cf_adjoint(const CfArgs& a, const double* gbar, double* scratch, double* ga, double* gb,
+                             hipStream_t s);
+
+// batched FUNC + dfunc (func.f90:1-142) in the reference's summation order
+struct FitArgs {
+    const double* x;  // [nfit][nx]
+    int32_t nfit, nx, norb, nbath;
+    const double* omega;
+    const double* dev;  // interleaved complex (norb, nw) column-major
+    int32_t nw, compat;
+    double* f;      // [nfit] or nullptr
+    double* g;      // [nfit][nx] or nullptr
+    double* work;   // scratch [nfit][norb][nw] complex residuals
+    const int32_t* active;  // optional [nfit]: bit 0 evaluate FUNC, bit 1 evaluate dfunc
+};
+hipError_t launch_fit(const FitArgs& a, hipStream_t s);
+
+}  // namespace moed

@@ -1,0 +1,789 @@
+// HIP kernels of libmoed for gfx950 (MI355X, CDNA4, wave64) and their launchers.
+//
+// Layout of a sector vector in HBM: x[iu + nU*id], up string rank iu fastest
+// (ed_basis.F90:120-123).  A "row" is the nU contiguous doubles of one down string.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "moed_internal.h"
+
+namespace moed {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// block sum in a fixed order (wave tree, then waves in index order); result valid in thread 0
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < NT / 64; ++w) s += red[w];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Complex arithmetic as the reference build evaluates it: textbook division, a real operand
+// promoted to (x, 0), no contraction (every product and sum rounded on its own).
+struct cxd {
+    double re, im;
+};
+__device__ __forceinline__ cxd cx_div(cxd a, cxd b) {
+#pragma clang fp contract(off)
+    const double den = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+}
+__device__ __forceinline__ cxd cx_rdiv(double r, cxd b) { return cx_div({r, 0.0}, b); }
+__device__ __forceinline__ cxd cx_mul(cxd a, cxd b) {
+#pragma clang fp contract(off)
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// |z| as sqrt(ax^2 + ay^2) followed by one correction step without FMA: C. F. Borges,
+// "An Improved Algorithm for hypot(a,b)", ACM TOMS 47 (2021), the fma-free corrected form.
+// Range guards keep ax^2 + ay^2 finite and normal; for the inputs of the bath fit the plain
+// path is taken.
+__device__ __forceinline__ double hypot_corrected(double x, double y) {
+#pragma clang fp contract(off)
+    x = fabs(x);
+    y = fabs(y);
+    const double ax = x < y ? y : x;
+    const double ay = x < y ? x : y;
+    const double EPS = 0x1p-54, SCALE = 0x1p-600;
+    auto kernel = [](double a, double b) {
+#pragma clang fp contract(off)
+        double h = sqrt(a * a + b * b);
+        double t1, t2;
+        if (h <= 2.0 * b) {
+            const double delta = h - b;
+            t1 = a * (2.0 * delta - a);
+            t2 = (delta - 2.0 * (a - b)) * delta;
+        } else {
+            const double delta = h - a;
+            t1 = 2.0 * delta * (a - 2.0 * b);
+            t2 = (4.0 * delta - b) * b + delta * delta;
+        }
+        h -= (t1 + t2) / (2.0 * h);
+        return h;
+    };
+    if (ax > 0x1p+511) {
+        if (ay <= ax * EPS) return ax + ay;
+        return kernel(ax * SCALE, ay * SCALE) / SCALE;
+    }
+    if (ay < 0x1p-511) {
+        if (ax >= ay / EPS) return ax + ay;
+        return kernel(ax / SCALE, ay / SCALE) * SCALE;
+    }
+    if (ay <= ax * EPS) return ax + ay;
+    return kernel(ax, ay);
+}
+
+__device__ __forceinline__ double round_f32(double v) { return (double)(float)v; }
+
+// ---------------------------------------------------------------- H v
+
+// One block owns R consecutive down strings (rows).  The R input rows are staged in LDS; the
+// up-string moves of a column are read once from the column tables and applied to all R rows
+// from LDS; down-string moves are row-uniform and read whole rows of x (coalesced); the
+// spin-flip and pair-exchange terms gather x at (row', column').  The diagonal is summed per
+// element in the reference's order (ed_hamiltonian.F90:110-139): onsite up sites (precomputed
+// per column), onsite down sites, U, then the inter-orbital pairs in (uu, ud, du, dd) order.
+template <int NORB, int R, int NT>
+__global__ void __launch_bounds__(NT) hv_kernel(HvArgs P) {
+    extern __shared__ double xs[];  // R * nU staged input rows
+    __shared__ double s_coef[kMaxCoef];
+    __shared__ double s_ek[16];
+    __shared__ int32_t s_dt[R][kMaxHop];
+    __shared__ double s_dc[R][kMaxHop];
+    __shared__ int32_t s_ndh[R];
+    __shared__ uint32_t s_sd[R];
+    __shared__ int32_t s_dimp[R][NORB * NORB > 0 ? NORB * NORB : 1];
+    __shared__ double s_red[NT / 64];
+
+    if (P.stop && *P.stop) return;
+    const int nU = P.up.count, nD = P.dn.count;
+    const int b = blockIdx.x;
+    const int grp = (b % kXcd) * P.rows_per_xcd + b / kXcd;  // XCD k owns a contiguous band of rows
+    const int row0 = grp * R;
+    if (row0 >= nD) return;
+    const int nr = min(R, nD - row0);
+    const int tid = threadIdx.x;
+
+    for (int k = tid; k < P.ncoef; k += NT) s_coef[k] = P.coef[k];
+    if (tid < P.nsite) s_ek[tid] = P.ek[tid];
+    if (tid < R) {
+        const int d = row0 + tid;
+        s_sd[tid] = tid < nr ? P.dn.str[d] : 0u;
+        s_ndh[tid] = tid < nr ? (int)P.dn.nhop[d] : 0;
+    }
+    for (int k = tid; k < R * kMaxHop; k += NT) {
+        const int r = k / kMaxHop, q = k % kMaxHop, d = row0 + r;
+        if (r < nr && q < P.dn.maxhop) {
+            const uint32_t e = P.dn.hop[(size_t)q * nD + d];
+            double c = P.coef[(e >> 16) & 0xff];
+            s_dt[r][q] = (int32_t)(e & 0xffffu);
+            s_dc[r][q] = (e >> 31) ? -c : c;
+        }
+    }
+    if (NORB > 1) {
+        for (int k = tid; k < R * NORB * NORB; k += NT) {
+            const int r = k / (NORB * NORB), p = k % (NORB * NORB), d = row0 + r;
+            s_dimp[r][p] = r < nr ? P.dn.imp[(size_t)p * nD + d] : -1;
+        }
+    }
+    const double* xrow0 = P.x + (size_t)row0 * nU;
+    for (int k = tid; k < nr * nU; k += NT) xs[k] = xrow0[k];
+    __syncthreads();
+
+    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J, J = P.J;
+    const double inv = P.yscale ? 1.0 / P.yscale[0] : 1.0;
+    double dsum = 0.0;
+    for (int iu = tid; iu < nU; iu += NT) {
+        const uint32_t su = P.up.str[iu];
+        const double Sup = P.up.onsite[iu];
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r < nr) {
+                const uint32_t sd = s_sd[r];
+                double S = Sup;
+                for (uint32_t m = sd; m; m &= m - 1) S += s_ek[__builtin_ctz(m)];
+#pragma unroll
+                for (int o = 0; o < NORB; ++o)
+                    if ((su >> o) & (sd >> o) & 1u) S += U;
+#pragma unroll
+                for (int o = 0; o < NORB; ++o)
+#pragma unroll
+                    for (int p = o + 1; p < NORB; ++p) {
+                        const uint32_t uo = (su >> o) & 1u, upp = (su >> p) & 1u;
+                        const uint32_t dO = (sd >> o) & 1u, dp = (sd >> p) & 1u;
+                        if (uo & upp) S += UmJ3;
+                        if (uo & dp) S += Um2J;
+                        if (dO & upp) S += Um2J;
+                        if (dO & dp) S += UmJ3;
+                    }
+                acc[r] = S * xs[r * nU + iu];
+            } else {
+                acc[r] = 0.0;
+            }
+        }
+        // up-string moves (hybridisation and impurity hopping of spin up)
+        const int nh = P.up.nhop[iu];
+        for (int k = 0; k < nh; ++k) {
+            const uint32_t e = P.up.hop[(size_t)k * nU + iu];
+            const int t = (int)(e & 0xffffu);
+            double c = s_coef[(e >> 16) & 0xff];
+            c = (e >> 31) ? -c : c;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r < nr) acc[r] += c * xs[r * nU + t];
+        }
+        // down-string moves: whole rows
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r < nr) {
+                const int ndh = s_ndh[r];
+                for (int k = 0; k < ndh; ++k) acc[r] += s_dc[r][k] * P.x[(size_t)s_dt[r][k] * nU + iu];
+            }
+        }
+        // spin flip (i,j): up j->i, down i->j; pair exchange (i,j): up j->i, down j->i.
+        // Element J * sign_up * sign_dn (the -J of ed_operators.F90:215,268 times the operator signs).
+        if (NORB > 1) {
+#pragma unroll
+            for (int i = 0; i < NORB; ++i)
+#pragma unroll
+                for (int j = 0; j < NORB; ++j) {
+                    if (i == j) continue;
+                    const int32_t mu = P.up.imp[(size_t)(i * NORB + j) * nU + iu];
+                    if (mu < 0) continue;
+                    const int uu = mu & 0xffff;
+                    const double cu = (mu >> 16) ? -J : J;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if (r >= nr) continue;
+                        const int32_t msf = s_dimp[r][j * NORB + i];
+                        if (msf >= 0) {
+                            const double c = (msf >> 16) ? -cu : cu;
+                            acc[r] += c * P.x[(size_t)(msf & 0xffff) * nU + uu];
+                        }
+                        const int32_t mpe = s_dimp[r][i * NORB + j];
+                        if (mpe >= 0) {
+                            const double c = (mpe >> 16) ? -cu : cu;
+                            acc[r] += c * P.x[(size_t)(mpe & 0xffff) * nU + uu];
+                        }
+                    }
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r < nr) {
+                const double v = P.yscale ? acc[r] * inv : acc[r];
+                P.y[(size_t)(row0 + r) * nU + iu] = v;
+                if (P.dot_part) dsum += xs[r * nU + iu] * v;
+            }
+        }
+    }
+    if (P.dot_part) {
+        const double s = block_sum<NT>(dsum, s_red);
+        if (tid == 0) P.dot_part[grp] = s;
+    }
+}
+
+template <int NORB, int R, int NT>
+static hipError_t hv_launch_t(const HvArgs& a, hipStream_t s) {
+    const int ngrp = (a.dn.count + R - 1) / R;
+    HvArgs p = a;
+    p.rows_per_xcd = (ngrp + kXcd - 1) / kXcd;
+    p.rows_per_block = R;
+    const int grid = p.rows_per_xcd * kXcd;
+    const size_t lds = (size_t)R * a.up.count * sizeof(double);
+    hipLaunchKernelGGL((hv_kernel<NORB, R, NT>), dim3(grid), dim3(NT), lds, s, p);
+    return hipGetLastError();
+}
+
+template <int NORB, int R>
+static hipError_t hv_launch_nt(const HvArgs& a, int threads, hipStream_t s) {
+    switch (threads) {
+        case 512:
+            return hv_launch_t<NORB, R, 512>(a, s);
+        case 1024:
+            return hv_launch_t<NORB, R, 1024>(a, s);
+        default:
+            return hv_launch_t<NORB, R, 256>(a, s);
+    }
+}
+
+template <int NORB>
+static hipError_t hv_launch_r(const HvArgs& a, int threads, hipStream_t s) {
+    switch (a.rows_per_block) {
+        case 2:
+            return hv_launch_nt<NORB, 2>(a, threads, s);
+        case 4:
+            return hv_launch_nt<NORB, 4>(a, threads, s);
+        default:
+            return hv_launch_nt<NORB, 1>(a, threads, s);
+    }
+}
+
+hipError_t launch_hv(const HvArgs& a, int norb, int threads, hipStream_t s) {
+    switch (norb) {
+        case 1:
+            return hv_launch_r<1>(a, threads, s);
+        case 2:
+            return hv_launch_r<2>(a, threads, s);
+        case 3:
+            return hv_launch_r<3>(a, threads, s);
+        case 4:
+            return hv_launch_r<4>(a, threads, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------- apply_c
+
+// y(out) = 0.0 + sgn * x(src) per target element (ed_operators.F90:44-58: vec_out(i) = vec_out(i) +
+// vec_all(j)*sgn with vec_out zeroed), y = 0 where the operator annihilates the target state.
+__global__ void apply_c_kernel(ApplyCArgs a) {
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int id = blockIdx.y;
+    if (iu >= a.nU_out) return;
+    double v = 0.0;
+    if (a.spin == 0) {
+        const int32_t m = a.map[iu];
+        if (m >= 0) {
+            const double xv = a.x[(size_t)(m & 0x3fffffff) + (size_t)a.nU_in * id];
+            v = 0.0 + ((m >> 30) & 1 ? -xv : xv);
+        }
+    } else {
+        const int32_t m = a.map[id];
+        if (m >= 0) {
+            const double xv = a.x[(size_t)iu + (size_t)a.nU_in * (m & 0x3fffffff)];
+            v = 0.0 + ((m >> 30) & 1 ? -xv : xv);
+        }
+    }
+    a.y[(size_t)iu + (size_t)a.nU_out * id] = v;
+}
+
+hipError_t launch_apply_c(const ApplyCArgs& a, hipStream_t s) {
+    dim3 grid((a.nU_out + 255) / 256, a.nD_out);
+    hipLaunchKernelGGL(apply_c_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fills
+
+__global__ void fill_start_kernel(uint64_t seed, int64_t n, double* x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = (seed ^ (uint64_t)i) + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        x[i] = 2.0 * u - 1.0;
+    }
+}
+
+hipError_t launch_fill_start(uint64_t seed, int64_t n, double* x, hipStream_t s) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_start_kernel, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, s, seed, n, x);
+    return hipGetLastError();
+}
+
+__global__ void fill_kernel(double v, int64_t n, double* x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+hipError_t launch_fill(double v, int64_t n, double* x, hipStream_t s) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, s, v, n, x);
+    return hipGetLastError();
+}
+
+__global__ void scale_copy_kernel(const double* x, double alpha, int64_t n, double* y) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = alpha * x[i];
+}
+
+hipError_t launch_scale_copy(const double* x, double alpha, int64_t n, double* y, hipStream_t s) {
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(scale_copy_kernel, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, s, x, alpha, n,
+                       y);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- reductions
+
+__global__ void __launch_bounds__(256) dot_partial_kernel(const double* a, const double* b, int64_t n, double* part) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += a[i] * b[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+hipError_t launch_dot_partial(const double* a, const double* b, int64_t n, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, a, b, n, part);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(1024) reduce_final_kernel(const double* part, int npart, double* out, int slot,
+                                                            int op, const double* divp) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 1024) s += part[i];
+    s = block_sum<1024>(s, red);
+    if (threadIdx.x == 0) {
+        if (op == 1) s = s / divp[0];
+        if (op == 2) s = sqrt(s);
+        out[slot] = s;
+    }
+}
+
+hipError_t launch_reduce_final(const double* part, int npart, double* out, int slot, int op, const double* divp,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(1024), 0, s, part, npart, out, slot, op, divp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- Lanczos (reference recurrence)
+
+__global__ void __launch_bounds__(256) lz_update_kernel(int j, const double* w, const double* u1, const double* u0,
+                                                        double* out, int64_t n, LzScal sc, double* part) {
+    __shared__ double red[4];
+    if (*sc.stop) return;
+    const double sb = sqrt(sc.B[j - 1]);
+    const double c1 = 1.0 / sb, a = sc.A[j - 1];
+    const double c3 = (j >= 2) ? -(sb / sqrt(sc.B[j - 2])) : 0.0;
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double v = (w[i] - a * u1[i]) * c1;
+        if (j >= 2) v += c3 * u0[i];
+        out[i] = v;
+        s += v * v;
+    }
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+hipError_t launch_lz_update(int j, const double* w, const double* u1, const double* u0, double* out, int64_t n,
+                            LzScal sc, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(lz_update_kernel, dim3(kRedBlocks), dim3(256), 0, s, j, w, u1, u0, out, n, sc, part);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(1024) lz_finish_b_kernel(int j, const double* part, int npart, LzScal sc) {
+    __shared__ double red[16];
+    if (*sc.stop) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 1024) s += part[i];
+    s = block_sum<1024>(s, red);
+    if (threadIdx.x == 0) sc.B[j] = s;
+}
+
+__global__ void __launch_bounds__(1024) lz_finish_a_kernel(int j, const double* part, int npart, LzScal sc,
+                                                           double thr) {
+    __shared__ double red[16];
+    if (*sc.stop) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 1024) s += part[i];
+    s = block_sum<1024>(s, red);
+    if (threadIdx.x == 0) {
+        sc.A[j] = s / sc.B[j];
+        if (j >= 2 && sc.B[j] < thr) {
+            *sc.stop = 1;
+            *sc.nsc = j;
+        }
+    }
+}
+
+hipError_t launch_lz_finish_b(int j, const double* part, int npart, LzScal sc, hipStream_t s) {
+    hipLaunchKernelGGL(lz_finish_b_kernel, dim3(1), dim3(1024), 0, s, j, part, npart, sc);
+    return hipGetLastError();
+}
+hipError_t launch_lz_finish_a(int j, const double* part, int npart, LzScal sc, double thr, hipStream_t s) {
+    hipLaunchKernelGGL(lz_finish_a_kernel, dim3(1), dim3(1024), 0, s, j, part, npart, sc, thr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- reorthogonalisation
+
+constexpr int kMdRows = 16;  // rows per thread in the tall-skinny kernels
+constexpr int kMdThreads = 256;
+constexpr int kMdBlockRows = kMdRows * kMdThreads;
+
+int multidot_blocks(int64_t n) { return (int)((n + kMdBlockRows - 1) / kMdBlockRows); }
+
+// part[i*nblk + blk] = sum over the block's rows of u_i * w (i < k1), part[k1*nblk + blk] = sum w^2
+__global__ void __launch_bounds__(kMdThreads) multidot_kernel(const double* V, int64_t ld, int k1, const double* w,
+                                                             int64_t n, double* part, int nblk) {
+    extern __shared__ double red[];  // (k1+1) * 4
+    const int64_t base = (int64_t)blockIdx.x * kMdBlockRows + threadIdx.x;
+    double wv[kMdRows];
+    double ww = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMdRows; ++q) {
+        const int64_t r = base + (int64_t)q * kMdThreads;
+        wv[q] = r < n ? w[r] : 0.0;
+        ww += wv[q] * wv[q];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i <= k1; ++i) {
+        double s;
+        if (i < k1) {
+            const double* u = V + (size_t)i * ld;
+            s = 0.0;
+#pragma unroll
+            for (int q = 0; q < kMdRows; ++q) {
+                const int64_t r = base + (int64_t)q * kMdThreads;
+                if (r < n) s += u[r] * wv[q];
+            }
+        } else {
+            s = ww;
+        }
+        s = wave_sum(s);
+        if (lane == 0) red[i * 4 + wid] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= k1; i += kMdThreads)
+        part[(size_t)i * nblk + blockIdx.x] = ((red[i * 4] + red[i * 4 + 1]) + red[i * 4 + 2]) + red[i * 4 + 3];
+}
+
+hipError_t launch_multidot(const double* V, int64_t ld, int k1, const double* w, int64_t n, double* part, int nblk,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(multidot_kernel, dim3(nblk), dim3(kMdThreads), (size_t)(k1 + 1) * 4 * sizeof(double), s, V, ld,
+                       k1, w, n, part, nblk);
+    return hipGetLastError();
+}
+
+// h[i] = sum_blk part[i*nblk + blk] for i <= k1, one block per i
+__global__ void __launch_bounds__(256) multidot_final_kernel(const double* part, int nblk, double* h) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += part[(size_t)blockIdx.x * nblk + b];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) h[blockIdx.x] = s;
+}
+
+hipError_t launch_multidot_final(const double* part, int k1, int nblk, double* h, hipStream_t s) {
+    hipLaunchKernelGGL(multidot_final_kernel, dim3(k1 + 1), dim3(256), 0, s, part, nblk, h);
+    return hipGetLastError();
+}
+
+// out = w - sum_i c_i u_i, part[blk] = block sum of out^2
+__global__ void __launch_bounds__(kMdThreads) project_out_kernel(const double* V, int64_t ld, int k1, const double* c,
+                                                                const double* w, double* out, int64_t n,
+                                                                double* part) {
+    extern __shared__ double sc[];
+    __shared__ double red[4];
+    for (int i = threadIdx.x; i < k1; i += kMdThreads) sc[i] = c[i];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kMdBlockRows + threadIdx.x;
+    double acc[kMdRows];
+#pragma unroll
+    for (int q = 0; q < kMdRows; ++q) {
+        const int64_t r = base + (int64_t)q * kMdThreads;
+        acc[q] = r < n ? w[r] : 0.0;
+    }
+    for (int i = 0; i < k1; ++i) {
+        const double* u = V + (size_t)i * ld;
+        const double ci = sc[i];
+#pragma unroll
+        for (int q = 0; q < kMdRows; ++q) {
+            const int64_t r = base + (int64_t)q * kMdThreads;
+            if (r < n) acc[q] -= ci * u[r];
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMdRows; ++q) {
+        const int64_t r = base + (int64_t)q * kMdThreads;
+        if (r < n) {
+            out[r] = acc[q];
+            s += acc[q] * acc[q];
+        }
+    }
+    s = block_sum<kMdThreads>(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+hipError_t launch_project_out(const double* V, int64_t ld, int k1, const double* c, const double* w, double* out,
+                              int64_t n, double* part, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(project_out_kernel, dim3(nblk), dim3(kMdThreads), (size_t)std::max(k1, 1) * sizeof(double), s,
+                       V, ld, k1, c, w, out, n, part);
+    return hipGetLastError();
+}
+
+// Y[:, i] = sum_k S[k + m*i] V[:, k]
+__global__ void __launch_bounds__(256) ritz_kernel(const double* V, int64_t ld, int m, const double* S, int nev,
+                                                   double* Y, int64_t ldy, int64_t n) {
+    extern __shared__ double sS[];
+    for (int i = threadIdx.x; i < m * nev; i += 256) sS[i] = S[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double v = V[(size_t)k * ld + r];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < nev) acc[i] += sS[k + m * i] * v;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (i < nev) Y[(size_t)i * ldy + r] = acc[i];
+}
+
+hipError_t launch_ritz(const double* V, int64_t ld, int m, const double* S, int nev, double* Y, int64_t ldy, int64_t n,
+                       hipStream_t s) {
+    if (nev > 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ritz_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), (size_t)m * nev * sizeof(double), s,
+                       V, ld, m, S, nev, Y, ldy, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- continued fraction
+
+// green_diag's continued fraction (ed_green.F90:160-168, 176-184) at one frequency per thread, with
+// z = cmplx(+-E, w) rounded as the reference's default-kind cmplx() when compat is set, and
+// G += weight * g in the reference's complex arithmetic.
+__global__ void cf_kernel(CfArgs a) {
+    extern __shared__ double sab[];  // a[0..n], b[0..n]
+    for (int k = threadIdx.x; k <= a.n; k += blockDim.x) {
+        sab[k] = a.a[k];
+        sab[a.n + 1 + k] = a.b[k];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nw) return;
+    const double* A = sab;
+    const double* B = sab + a.n + 1;
+    const double E = a.sign > 0 ? a.E : -a.E;
+    const cxd z = {a.compat ? round_f32(E) : E, a.compat ? round_f32(a.omega[i]) : a.omega[i]};
+    cxd g;
+    if (a.sign > 0) {
+        g = cx_rdiv(B[a.n], {z.re - A[a.n], z.im});
+        for (int j = a.n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re - A[j]) - g.re, z.im - g.im});
+    } else {
+        g = cx_rdiv(B[a.n], {z.re + A[a.n], z.im});
+        for (int j = a.n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re + A[j]) - g.re, z.im - g.im});
+    }
+    double* G = a.G + (size_t)2 * i * a.stride;
+    const cxd wg = cx_mul({a.weight, 0.0}, g);
+    G[0] = G[0] + wg.re;
+    G[1] = G[1] + wg.im;
+}
+
+hipError_t launch_cf(const CfArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(cf_kernel, dim3((a.nw + 255) / 256), dim3(256), (size_t)2 * (a.n + 1) * sizeof(double), s, a);
+    return hipGetLastError();
+}
+
+// Adjoint of the continued fraction.  With D_j = z -+ a_j - t_{j+1}, t_j = b_j / D_j, g = t_0:
+// dg/dt_j = P_j = prod_{k<j} t_k / D_k, dg/db_j = P_j / D_j, dg/da_j = +-P_j t_j / D_j.
+// For real theta: dL/dtheta = Re(conj(gbar) dg/dtheta), gbar = dL/dRe g + i dL/dIm g.
+// contrib[(j)*nw + i] per frequency, then summed over frequencies in index order.
+__global__ void cf_adjoint_kernel(CfArgs a, const double* gbar, double* scratch, double* ca, double* cb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nw) return;
+    const int n = a.n, nw = a.nw;
+    const double E = a.sign > 0 ? a.E : -a.E;
+    const cxd z = {a.compat ? round_f32(E) : E, a.compat ? round_f32(a.omega[i]) : a.omega[i]};
+    const double sgn = a.sign > 0 ? -1.0 : 1.0;  // D_j = z + sgn*a_j - t_{j+1}
+    // scratch holds D_j and t_j per frequency: [(j*nw + i)*4]
+    cxd t = {0.0, 0.0};
+    for (int j = n; j >= 0; --j) {
+        const cxd D = {(z.re + sgn * a.a[j]) - t.re, z.im - t.im};
+        t = cx_rdiv(a.b[j], D);
+        double* sp = scratch + ((size_t)j * nw + i) * 4;
+        sp[0] = D.re;
+        sp[1] = D.im;
+        sp[2] = t.re;
+        sp[3] = t.im;
+    }
+    const cxd gb = {gbar[2 * i], gbar[2 * i + 1]};
+    cxd P = {1.0, 0.0};
+    for (int j = 0; j <= n; ++j) {
+        const double* sp = scratch + ((size_t)j * nw + i) * 4;
+        const cxd D = {sp[0], sp[1]}, tj = {sp[2], sp[3]};
+        const cxd dgb = cx_div(P, D);                          // dg/db_j
+        const cxd dga = cx_mul({-sgn, 0.0}, cx_div(cx_mul(P, tj), D));  // dg/da_j
+        ca[(size_t)j * nw + i] = gb.re * dga.re + gb.im * dga.im;
+        cb[(size_t)j * nw + i] = gb.re * dgb.re + gb.im * dgb.im;
+        P = cx_div(cx_mul(P, tj), D);
+    }
+}
+
+__global__ void __launch_bounds__(256) sum_rows_kernel(const double* c, int nw, double* out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nw; i += 256) s += c[(size_t)blockIdx.x * nw + i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+hipError_t launch_cf_adjoint(const CfArgs& a, const double* gbar, double* scratch, double* ga, double* gb,
+                             hipStream_t s) {
+    const size_t nw = a.nw, n1 = a.n + 1;
+    double* ca = scratch + n1 * nw * 4;
+    double* cbp = ca + n1 * nw;
+    hipLaunchKernelGGL(cf_adjoint_kernel, dim3((a.nw + 255) / 256), dim3(256), 0, s, a, gbar, scratch, ca, cbp);
+    hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)n1), dim3(256), 0, s, ca, a.nw, ga);
+    hipLaunchKernelGGL(sum_rows_kernel, dim3((unsigned)n1), dim3(256), 0, s, cbp, a.nw, gb);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- bath fit (FUNC + dfunc)
+
+// x packing of ev_to_x (func.f90:165-191): x[0:nbath] bath energies, x[nbath + o*nbath + k] = V(o, k),
+// x[nx-norb+o] = ef(o).
+__device__ __forceinline__ cxd fit_delta(const double* x, int nx, int norb, int nbath, int o, cxd z) {
+    // Delta_o(i w) = sum_k V(o,k)^2 / (z - e_k) + ef_o  (func.f90:19-29), in the reference's order
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nbath; ++k) {
+        const double v = x[nbath + o * nbath + k];
+        const cxd q = cx_rdiv(v * v, {z.re - x[k], z.im});
+        re = re + q.re;
+        im = im + q.im;
+    }
+    return {re + x[nx - norb + o], im};
+}
+
+// pass 1: residuals r_o(w) = Delta_o - Dev_o for every (fit, w), one thread each
+__global__ void fit_residual_kernel(FitArgs a) {
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (iw >= a.nw) return;
+    if (a.active && !a.active[f]) return;
+    const double* x = a.x + (size_t)f * a.nx;
+    const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
+    for (int o = 0; o < a.norb; ++o) {
+        const cxd d = fit_delta(x, a.nx, a.norb, a.nbath, o, z);
+        const size_t io = (size_t)o + (size_t)a.norb * iw;
+        double* r = a.work + ((size_t)f * a.norb * a.nw + io) * 2;
+        r[0] = d.re - a.dev[2 * io];
+        r[1] = d.im - a.dev[2 * io + 1];
+    }
+}
+
+// pass 2: one thread per (fit, component) summing sequentially in the reference's loop order.
+// component 0: FUNC (func.f90:30-42); 1..nbath: d/de_k (func.f90:89-101); then d/dV (:103-116),
+// then d/def (:118-124); all divided as func.f90:131.
+__global__ void fit_sum_kernel(FitArgs a) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.y;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > a.nx) return;
+    if (a.active && !(a.active[f] & (c == 0 ? 1 : 2))) return;
+    if (c == 0 && !a.f) return;
+    if (c > 0 && !a.g) return;
+    const int norb = a.norb, nbath = a.nbath, nw = a.nw, nx = a.nx;
+    const double* x = a.x + (size_t)f * nx;
+    const double* R = a.work + (size_t)f * norb * nw * 2;
+    double s = 0.0;
+    if (c == 0) {
+        for (int iw = 0; iw < nw; ++iw)
+            for (int o = 0; o < norb; ++o) {
+                const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
+                const double h = hypot_corrected(r[0], r[1]);
+                s = s + h * h / a.omega[iw];
+            }
+        a.f[f] = s / (double)(float)(norb * norb * nw);
+        return;
+    }
+    const int p = c - 1;
+    if (p < nbath) {
+        const int k = p;
+        for (int o = 0; o < norb; ++o) {
+            const double v = x[nbath + o * nbath + k];
+            for (int iw = 0; iw < nw; ++iw) {
+                const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
+                const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
+                const cxd den = {z.re - x[k], z.im};
+                const cxd cr = {r[0], -r[1]};
+                const cxd t = cx_div(cx_div(cx_mul(cx_mul(cr, {v, 0.0}), {v, 0.0}), den), den);
+                s = s + 2.0 * t.re / a.omega[iw];
+            }
+        }
+    } else if (p < nx - norb) {
+        const int o = (p - nbath) / nbath, k = (p - nbath) % nbath;
+        const double v = x[p];
+        for (int iw = 0; iw < nw; ++iw) {
+            const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
+            const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
+            const cxd den = {z.re - x[k], z.im};
+            const cxd cr = {r[0], -r[1]};
+            const cxd q = cx_div(cx_mul(cr, {v, 0.0}), den);
+            const double tre = q.re + q.re;
+            s = s + 2.0 * tre / a.omega[iw];
+        }
+    } else {
+        const int o = p - (nx - norb);
+        for (int iw = 0; iw < nw; ++iw) {
+            const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
+            s = s + 2.0 * r[0] / a.omega[iw];
+        }
+    }
+    a.g[(size_t)f * nx + p] = s / (double)(float)(norb * norb) / (double)(float)nw;
+}
+
+hipError_t launch_fit(const FitArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(fit_residual_kernel, dim3((a.nw + 127) / 128, a.nfit), dim3(128), 0, s, a);
+    hipLaunchKernelGGL(fit_sum_kernel, dim3((a.nx + 1 + 63) / 64, a.nfit), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace moed

@@ -1,0 +1,150 @@
+! Driver for the reference build (oracle/_ref/ref_driver).
+!
+! TEST INFRASTRUCTURE ONLY: it links the reference's own modules, compiled from the
+! reference sources by oracle/Makefile, and runs one of their routines on inputs
+! read from a binary stream file, writing the outputs to another.  It is used in
+! the build container to pin the C oracle and to write golden fixtures.
+!
+! Input file (stream, little endian):
+!   int32  task, norb, nbath, ne_up, ne_dn, i1, i2, i3
+!   real64 U, J, ek(nsite), vk(norb*nbath) column-major
+!   payload (task dependent)
+! Tasks:
+!   1 multiply_H  (ed_hamiltonian.F90:88)   payload x(D)            -> y(D)
+!   2 lanczos     (ed_lanczos.F90:14)       i1=nstep, payload v(D)  -> nstep_calc, a(0:nstep), b(0:nstep)
+!   3 apply_c     (ed_operators.F90:10)     i1=pm,i2=iorb,i3=ispin, payload v(D) -> D_out, v_out
+!   4 FUNC+dfunc  (func.f90:1,47)           i1=nw, payload x(nx), omega(nw), Dev(2*norb*nw) -> f, g(nx)
+!   5 minimization(func.f90:144)            same payload -> fret, x(nx)
+!   6 SORT        (func.f90:306)            i1=n, payload a(n) -> indx(n)
+!   7 H columns   (multiply_H on unit vectors) -> for each column: nnz, rows(nnz), vals(nnz)
+!   8 timing      i1 = calls; every MPI rank owns its row block -> seconds per call, ranks, y(1)
+program ref_driver
+    use parallel_params
+    use ed_config
+    use ed_basis
+    use ed_operators
+    use ed_hamiltonian
+    use ed_lanczos
+    implicit none
+    character(len=512) :: fin, fout
+    integer :: hdr(8), task, ne_up, ne_dn, i1, i2, i3, i, j, nnz, nstep_calc, nx, nwl
+    double precision :: uj(2), fret, t0, t1
+    double precision, allocatable :: x(:), y(:), a(:), b(:), om(:), dv(:), g(:)
+    double complex, allocatable :: dev(:,:)
+    integer, allocatable :: indx(:), rows(:)
+    type(basis_t) :: bas, bout
+    double precision, external :: func
+
+    call MPI_Init(ierr)
+    call MPI_Comm_size(comm, Nodes, ierr)
+    call MPI_Comm_rank(comm, Node, ierr)
+    call get_command_argument(1, fin)
+    call get_command_argument(2, fout)
+    open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(10) hdr
+    task = hdr(1); Norb = hdr(2); Nbath = hdr(3); ne_up = hdr(4); ne_dn = hdr(5)
+    i1 = hdr(6); i2 = hdr(7); i3 = hdr(8)
+    Nsite = Norb + Nbath
+    read(10) uj
+    U = uj(1); Jex = uj(2)
+    allocate(ek(nsite), vk(norb, norb+1:nsite))
+    read(10) ek
+    read(10) vk
+    if (node .eq. 0) open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+
+    select case (task)
+    case (1)
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%ntot), y(bas%ntot))
+        read(10) x
+        call multiply_H(bas, x, y)
+        write(11) y
+    case (2)
+        Nstep = i1
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%ntot), a(0:nstep), b(0:nstep))
+        read(10) x
+        call lanczos_iteration(bas, x, nstep_calc, a, b)
+        write(11) nstep_calc
+        write(11) a
+        write(11) b
+    case (3)
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%ntot))
+        read(10) x
+        call apply_c(bas, x, i1, i2, i3, bout, y)
+        write(11) bout%ntot
+        write(11) y
+    case (4, 5)
+        nwl = i1
+        nx = Nbath + Nbath*Norb + Norb
+        allocate(x(nx), g(nx), om(nwl), dv(2*Norb*nwl), dev(Norb, nwl))
+        read(10) x
+        read(10) om
+        read(10) dv
+        do i = 1, nwl
+            do j = 1, Norb
+                dev(j, i) = dcmplx(dv(2*(j-1+Norb*(i-1))+1), dv(2*(j-1+Norb*(i-1))+2))
+            enddo
+        enddo
+        if (task .eq. 4) then
+            fret = func(x, Nsite, nwl, nx, om, dev, Nbath, Norb, comm)
+            call dfunc(x, g, nwl, Nsite, nx, om, dev, Nbath, Norb, comm)
+            write(11) fret
+            write(11) g
+        else
+            call minimization(x, nwl, Nsite, nx, om, dev, Nbath, Norb, comm, fret)
+            write(11) fret
+            write(11) x
+        endif
+    case (6)
+        allocate(a(i1), indx(i1))
+        read(10) a
+        call sort(i1, a, indx)
+        write(11) indx
+    case (8)
+        ! time i1 calls of multiply_H after one untimed call; every rank owns nloc rows of the
+        ! sector and gathers the full vector each call, as in the reference run
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%ntot), y(bas%ntot))
+        do i = 1, bas%nloc
+            x(i) = 1.0d0 / dble(i + bas%offsets(node))
+        enddo
+        call multiply_H(bas, x, y)
+        call mpi_barrier(comm, ierr)
+        t0 = mpi_wtime()
+        do j = 1, i1
+            call multiply_H(bas, x, y)
+        enddo
+        call mpi_barrier(comm, ierr)
+        t1 = mpi_wtime()
+        if (node .eq. 0) then
+            write(11) (t1 - t0) / dble(i1)
+            write(11) nodes
+            write(11) y(1)
+        endif
+    case (7)
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%ntot), y(bas%ntot), rows(bas%ntot))
+        x = 0.0d0
+        do j = 1, bas%ntot
+            x(j) = 1.0d0
+            call multiply_H(bas, x, y)
+            x(j) = 0.0d0
+            nnz = 0
+            do i = 1, bas%ntot
+                if (y(i) .ne. 0.0d0) then
+                    nnz = nnz + 1
+                    rows(nnz) = i
+                    y(nnz) = y(i)
+                endif
+            enddo
+            write(11) nnz
+            write(11) rows(1:nnz)
+            write(11) y(1:nnz)
+        enddo
+    end select
+    if (node .eq. 0) close(11)
+    close(10)
+    call MPI_Finalize(ierr)
+end program ref_driver

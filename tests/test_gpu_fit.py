@@ -1,0 +1,80 @@
+"""GPU parity of the batched bath fit: FUNC/dfunc (func.f90) in the reference's operation order,
+calc_dev, and the CG minimiser (frprmn.f90) against the reference build's trajectory."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from moed import EDSolver
+from moed.model import omega_grid
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def fit():
+    z = np.load(os.path.join(GOLD, "ref_fit.npz"))
+    norb, nbath = int(z["norb"]), int(z["nbath"])
+    s = EDSolver(norb, nbath, 0.0, 0.0, np.zeros(norb + nbath), np.zeros((norb, nbath)))
+    yield z, s, norb, nbath
+    s.close()
+
+
+def test_func_dfunc_bit_exact_with_reference(fit):
+    z, s, norb, nbath = fit
+    f, g = s.func_dfunc(z["x0"], z["omega"], z["Dev"])
+    assert f[0] == float(z["f"])
+    assert np.array_equal(g[0], z["g"])
+
+
+def test_func_dfunc_batched_equals_oracle(fit):
+    z, s, norb, nbath = fit
+    rng = np.random.default_rng(1)
+    X = z["x0"][None, :] + 0.05 * rng.standard_normal((37, len(z["x0"]))) * (z["x0"] != 0)
+    f, g = s.func_dfunc(X, z["omega"], z["Dev"])
+    for i in range(len(X)):
+        assert f[i] == oracle.func(X[i], z["omega"], z["Dev"], norb, nbath)
+        assert np.array_equal(g[i], oracle.dfunc(X[i], z["omega"], z["Dev"], norb, nbath))
+
+
+def test_calc_dev_bit_exact(fit):
+    z, s, norb, nbath = fit
+    ef, ek, vk = oracle.x_to_ev(z["x0"], norb, nbath)
+    D = s.calc_dev(ek, vk, ef, z["omega"])
+    Do = oracle.calc_dev(oracle.Model(norb, nbath, 0, 0, ek, vk), ef, z["omega"])
+    assert np.array_equal(D, Do)
+
+
+def test_minimization_reproduces_reference_trajectory(fit):
+    z, s, norb, nbath = fit
+    x, fret, it, st = s.minimization(z["x0"], z["omega"], z["Dev"])
+    print("fret", fret[0], "ref", float(z["fret"]), "iters", it[0], flush=True)
+    assert st[0] == 0
+    assert np.max(np.abs(x[0] - z["xmin"])) <= 1e-8
+    assert abs(fret[0] - float(z["fret"])) <= 1e-8 * max(1e-12, abs(float(z["fret"]))) + 1e-20
+
+
+def test_minimization_batched_independent_starts(fit):
+    z, s, norb, nbath = fit
+    rng = np.random.default_rng(3)
+    X = z["x0"][None, :] + 0.02 * rng.standard_normal((6, len(z["x0"]))) * (z["x0"] != 0)
+    X[0] = z["x0"]
+    x, fret, it, st = s.minimization(X, z["omega"], z["Dev"])
+    for i in range(len(X)):
+        rc, xo, fo, ito, _ = oracle.frprmn(X[i], z["omega"], z["Dev"], norb, nbath)
+        assert st[i] == rc == 0
+        assert it[i] == ito and np.max(np.abs(x[i] - xo)) <= 1e-8
+
+
+def test_exact_fit_recovers_parameters():
+    norb, nbath = 1, 2
+    om = omega_grid(128, 20.0)
+    xt = np.array([-0.7, 0.9, 0.5, 0.4, 0.1])
+    with EDSolver(norb, nbath, 0.0, 0.0, np.zeros(3), np.zeros((1, 2))) as s:
+        ef, ek, vk = oracle.x_to_ev(xt, norb, nbath)
+        Dev = s.calc_dev(ek, vk, ef, om)
+        x, fret, it, st = s.minimization(xt + np.array([0.05, -0.05, 0.04, -0.03, 0.02]), om, Dev)
+    assert st[0] == 0 and fret[0] < 1e-12
+    assert np.allclose(np.abs(x[0]), np.abs(xt), atol=1e-4)

@@ -1,0 +1,185 @@
+"""GPU parity of the sector H v and apply_c kernels against the oracle (bit-exact matrix elements,
+product within rounding) and size-independent properties at the 14-site sector."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from moed import EDSolver
+from moed.model import load_configs, start_vector
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def omodel(norb, nbath, U, J, ek, vk, tij=None):
+    return oracle.Model(norb, nbath, U, J, ek, vk, tij)
+
+
+def random_model(rng, norb, nbath, tij=False, dense_v=False):
+    ns = norb + nbath
+    ek = rng.uniform(-2, 2, ns)
+    vk = rng.uniform(0.1, 1.0, (norb, nbath))
+    if not dense_v:  # block-diagonal coupling as in the workloads
+        nb = max(1, nbath // norb)
+        mask = np.zeros((norb, nbath), bool)
+        for k in range(nbath):
+            mask[min(k // nb, norb - 1), k] = True
+        vk = np.where(mask, vk, 0.0)
+    t = None
+    if tij:
+        t = rng.uniform(-0.3, 0.3, (norb, norb))
+        t = t + t.T
+        np.fill_diagonal(t, 0.0)
+    return ek, vk, t
+
+
+def gpu_columns(s, nu, nd, D):
+    cols = []
+    for j in range(D):
+        e = np.zeros(D)
+        e[j] = 1.0
+        y = s.multiply_H(nu, nd, e)
+        nz = np.nonzero(y)[0]
+        cols.append((nz.astype(np.int64), y[nz]))
+    return cols
+
+
+CASES = [
+    # norb, nbath, nu, nd, tij, dense_v
+    (2, 4, 3, 2, False, False),
+    (1, 5, 3, 3, False, False),
+    (3, 3, 3, 4, True, False),
+    (4, 2, 3, 3, True, False),
+    (2, 3, 2, 2, False, True),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_matrix_elements_bit_exact(case):
+    norb, nbath, nu, nd, tij, dense_v = case
+    print("case", case, flush=True)
+    rng = np.random.default_rng(hash(case) % 2**32)
+    ek, vk, t = random_model(rng, norb, nbath, tij, dense_v)
+    m = omodel(norb, nbath, 3.5, 0.45, ek, vk, t)
+    b = oracle.Basis(norb + nbath, nu, nd)
+    with EDSolver(norb, nbath, 3.5, 0.45, ek, vk, tij=t) as s:
+        up, dn = s.generate_basis(nu, nd)
+        assert np.array_equal(up, b.up) and np.array_equal(dn, b.dn)
+        cols = gpu_columns(s, nu, nd, b.ntot)
+    ref = oracle.h_rows(m, b)  # H is exactly symmetric: row j = column j
+    assert oracle.h_digest(cols) == oracle.h_digest(ref)
+
+
+def test_matrix_elements_match_reference_build_8site(configs):
+    c = configs["ns8_plain"]
+    z = np.load(os.path.join(GOLD, "ref_ns8.npz"))
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        D = s.dim(nu, nd)
+        cols = gpu_columns(s, nu, nd, D)
+    assert oracle.h_digest(cols) == str(z["h_digest"])
+    diag = np.array([v[r == j][0] if np.any(r == j) else 0.0 for j, (r, v) in enumerate(cols)])
+    assert np.array_equal(diag, z["h_diag"])
+
+
+def test_hv_product_8site_all_launch_shapes(configs):
+    c = configs["ns8_plain"]
+    z = np.load(os.path.join(GOLD, "ref_ns8.npz"))
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        x = start_vector(int(z["x_seed"]), s.dim(nu, nd))
+        ys = []
+        for threads in (256, 512, 1024):
+            for rows in (1, 2, 4):
+                s.set_hv_shape(threads, rows)
+                ys.append(s.multiply_H(nu, nd, x))
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0])  # every element's terms are summed in the same order
+    ref = z["hv_y"]
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(ys[0] - ref)) <= 1e-14 * scale
+
+
+def test_hv_12site_against_oracle_and_reference(configs):
+    c = configs["ns12_plain"]
+    z = np.load(os.path.join(GOLD, "ref_ns12.npz"))
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        x = start_vector(int(z["x_seed"]), s.dim(nu, nd))
+        y = s.multiply_H(nu, nd, x)
+    m = omodel(c.norb, c.nbath, c.U, c.J, c.ek, c.vk)
+    yo = oracle.multiply_H(m, oracle.Basis(c.nsite, nu, nd), x)
+    scale = np.abs(yo).max()
+    assert np.max(np.abs(y - yo)) <= 1e-14 * scale
+    assert np.max(np.abs(y[z["idx"]] - z["y_at"])) <= 1e-14 * scale
+
+
+def test_hv_14site_properties_and_oracle(configs):
+    """full-size sector (D = 11,778,624): symmetry, linearity and the oracle's product"""
+    c = configs["ns14_roofline"]
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        D = s.dim(nu, nd)
+        x = start_vector(1, D)
+        y = start_vector(2, D)
+        Hx = s.multiply_H(nu, nd, x)
+        Hy = s.multiply_H(nu, nd, y)
+        Hxy = s.multiply_H(nu, nd, 0.5 * x - 2.0 * y)
+    assert abs(np.dot(y, Hx) - np.dot(Hy, x)) <= 1e-12 * np.linalg.norm(Hx) * np.linalg.norm(y)
+    assert np.max(np.abs(Hxy - (0.5 * Hx - 2.0 * Hy))) <= 1e-12 * np.abs(Hx).max()
+    m = omodel(c.norb, c.nbath, c.U, c.J, c.ek, c.vk)
+    yo = oracle.multiply_H(m, oracle.Basis(c.nsite, nu, nd), x)
+    assert np.max(np.abs(Hx - yo)) <= 1e-14 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("case", [(2, 6, 4, 4), (3, 3, 3, 2), (1, 4, 0, 2), (2, 4, 6, 5)])
+def test_apply_c_bit_exact(case):
+    norb, nbath, nu, nd = case
+    print("case", case, flush=True)
+    rng = np.random.default_rng(sum(case))
+    ek, vk, _ = random_model(rng, norb, nbath)
+    ns = norb + nbath
+    b = oracle.Basis(ns, nu, nd)
+    x = rng.standard_normal(b.ntot)
+    with EDSolver(norb, nbath, 2.0, 0.3, ek, vk) as s:
+        for pm in (1, 2):
+            for iorb in range(1, norb + 1):
+                for ispin in (1, 2):
+                    du, dd = (1, 0) if ispin == 1 else (0, 1)
+                    sg = 1 if pm == 1 else -1
+                    if not (0 <= nu + sg * du <= ns and 0 <= nd + sg * dd <= ns):
+                        with pytest.raises(Exception):
+                            s.apply_c(nu, nd, x, pm, iorb, ispin)
+                        continue
+                    _, out = s.apply_c(nu, nd, x, pm, iorb, ispin)
+                    _, ref = oracle.apply_c(b, x, pm, iorb, ispin)
+                    assert np.array_equal(out, ref), (pm, iorb, ispin)
+
+
+def test_apply_c_matches_reference_build(configs):
+    c = configs["ns8_plain"]
+    z = np.load(os.path.join(GOLD, "ref_ns8.npz"))
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        x = start_vector(int(z["x_seed"]), s.dim(nu, nd))
+        for pm in (1, 2):
+            for iorb in (1, 2):
+                for ispin in (1, 2):
+                    _, out = s.apply_c(nu, nd, x, pm, iorb, ispin)
+                    assert np.array_equal(out, z[f"applyc_{pm}{iorb}{ispin}"])
+
+
+def test_bad_arguments_raise():
+    rng = np.random.default_rng(0)
+    ek, vk, _ = random_model(rng, 2, 2)
+    with EDSolver(2, 2, 1.0, 0.1, ek, vk) as s:
+        with pytest.raises(Exception):
+            s.multiply_H(5, 1, np.zeros(4))
+        with pytest.raises(Exception):
+            s.apply_c(2, 2, np.zeros(36), 3, 1, 1)
+        with pytest.raises(Exception):
+            s.lanczos_iteration(2, 2, np.zeros(36), 10)  # zero start vector
+    with pytest.raises(Exception):
+        EDSolver(2, 2, 1.0, 0.1, np.array([0, 0, np.nan, 0]), vk)

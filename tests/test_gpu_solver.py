@@ -1,0 +1,197 @@
+"""GPU parity of lanczos_iteration, ed_solve, ed_calc_green_ftn and the continued fraction
+against the oracle, dense ED and the committed golden energies."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import dense_ed
+from moed import EDSolver, MOED_F_EXACT_CMPLX
+from moed.model import load_configs, omega_grid, start_vector
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def omodel(c):
+    return oracle.Model(c.norb, c.nbath, c.U, c.J, c.ek, c.vk, c.tij)
+
+
+def test_lanczos_matches_reference_recurrence(configs):
+    c = configs["ns8_plain"]
+    z = np.load(os.path.join(GOLD, "ref_ns8.npz"))
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        x = start_vector(int(z["x_seed"]), s.dim(nu, nd))
+        nsc, a, b = s.lanczos_iteration(nu, nd, x, c.nstep)
+    assert nsc == int(z["lz_nstep_calc"])
+    # the recurrence without reorthogonalisation amplifies rounding once Ritz values converge:
+    # the early coefficients agree to rounding, and the continued fractions agree to 1e-8
+    assert np.allclose(a[:25], z["lz_a"][:25], rtol=1e-10, atol=1e-12)
+    assert np.allclose(b[1:25], z["lz_b"][1:25], rtol=1e-10)
+    assert b[0] == 0.0
+    om = omega_grid(c.nw, c.beta)
+    bb = b.copy()
+    bb[0] = 1.0
+    br = z["lz_b"].copy()
+    br[0] = 1.0
+    g = oracle.green_cf(+1, -10.0, nsc, a, bb, om)
+    gr = oracle.green_cf(+1, -10.0, int(z["lz_nstep_calc"]), z["lz_a"], br, om)
+    assert np.max(np.abs(g - gr)) <= 1e-8 * np.max(np.abs(gr))
+
+
+def test_lanczos_breakdown_in_small_krylov_space():
+    rng = np.random.default_rng(12)
+    ek, vk = rng.uniform(-1, 1, 4), rng.uniform(0.3, 0.9, (2, 2))
+    m = oracle.Model(2, 2, 2.0, 0.3, ek, vk)
+    b = oracle.Basis(4, 1, 1)
+    H = np.array([oracle.multiply_H(m, b, e) for e in np.eye(b.ntot)]).T
+    w, V = np.linalg.eigh(H)
+    v = V[:, [0, 5, 9]] @ np.array([0.6, -0.3, 0.5])
+    with EDSolver(2, 2, 2.0, 0.3, ek, vk) as s:
+        nsc, a, bb = s.lanczos_iteration(1, 1, v, 40)
+    nsco, ao, bo = oracle.lanczos(m, b, v, 40)
+    assert nsc == nsco == 3
+    assert np.all(a[4:] == 0) and np.all(bb[4:] == 0)
+    assert np.allclose(a[:3], ao[:3], rtol=1e-12)
+
+
+def test_solve_ground_state_8site_against_dense(configs):
+    c = configs["ns8_plain"]
+    e0 = json.load(open(os.path.join(GOLD, "e0.json")))["ns8_plain"]
+    with EDSolver.from_config(c, nev=2) as s:
+        kept = s.ed_solve()
+        ev = s.sector_eigenvalues(1)
+        vec = s.eigvec(1, 1)
+    E0 = float.fromhex(e0["E0"])
+    assert abs(ev[0] - E0) <= 1e-10 * abs(E0)
+    assert abs(ev[1] - float.fromhex(e0["E1"])) <= 1e-10 * abs(E0)
+    assert kept[0][0] == ev[0] and kept[0][2:] == (1, 1)
+    # Ritz vector residual
+    m = omodel(c)
+    r = oracle.multiply_H(m, oracle.Basis(c.nsite, *c.sectors[0]), vec) - ev[0] * vec
+    assert np.linalg.norm(r) < 1e-9 and abs(np.linalg.norm(vec) - 1) < 1e-12
+
+
+def test_solve_ground_state_12site_against_arpack(configs):
+    c = configs["ns12_plain"]
+    e0 = json.load(open(os.path.join(GOLD, "e0.json"))).get("ns12_plain")
+    if e0 is None:
+        pytest.skip("no ARPACK golden energy for the 12-site sector")
+    with EDSolver.from_config(c) as s:
+        s.ed_solve()
+        ev = s.sector_eigenvalues(1)
+    E0 = float.fromhex(e0["E0"])
+    print("E0 gpu", repr(ev[0]), "arpack", repr(E0), flush=True)
+    assert abs(ev[0] - E0) <= 1e-10 * abs(E0)
+
+
+def test_solve_all_sectors_small_model_and_selection():
+    """every sector of a 5-site model (dims 1..100): eigenvalues vs dense ED, state selection
+    vs the oracle's restatement of ed_solve (SORT + boltzmann_factor)"""
+    rng = np.random.default_rng(21)
+    norb, nbath = 2, 3
+    ns = norb + nbath
+    ek, vk = rng.uniform(-1.5, 1.5, ns), rng.uniform(0.2, 0.8, (norb, nbath))
+    U, J, beta, nev = 3.0, 0.4, 5.0, 4
+    sectors = [(a, b) for a in range(ns + 1) for b in range(ns + 1)]
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=beta, nev=nev, nstep=30, sectors=sectors) as s:
+        kept = s.ed_solve()
+        evs = [s.sector_eigenvalues(i + 1) for i in range(len(sectors))]
+    allv = []
+    for (nu, nd), ev in zip(sectors, evs):
+        sec = dense_ed.Sector(ns, nu, nd)
+        w = np.linalg.eigvalsh(dense_ed.hamiltonian(sec, norb, nbath, U, J, ek, vk).toarray())
+        k = min(nev, len(w))
+        assert np.allclose(ev[:k], w[:k], rtol=0, atol=1e-10 * max(1, abs(w[0])))
+        assert np.all(ev[k:] == np.finfo(float).max)
+        allv.extend(ev)
+    ref = oracle.select_states(np.array(allv), nev, beta)
+    assert len(kept) == len(ref)
+    for k, r in zip(kept, ref):
+        assert k[2:] == r[2:] and abs(k[0] - r[0]) < 1e-10 and abs(k[1] - r[1]) < 1e-9
+
+
+def test_green_8site_against_oracle(configs):
+    c = configs["ns8_plain"]
+    nu, nd = c.sectors[0]
+    om = omega_grid(c.nw, c.beta)
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+        vec = s.eigvec(1, 1)
+        recs = s.green_records()
+    m = omodel(c)
+    b = oracle.Basis(c.nsite, nu, nd)
+    Gr = np.zeros((c.norb, c.nw), complex)
+    for (E, p, isec, ilev) in kept:
+        for iorb in range(1, c.norb + 1):  # N_up == N_dn: spin up only, factor 1 (ed_green.F90:87-94)
+            g, _, _ = oracle.green_diag(m, b, vec, E, p, 1.0, iorb, 1, c.nstep, om)
+            Gr[iorb - 1] += g
+    err = np.max(np.abs(G - Gr)) / np.max(np.abs(Gr))
+    print("G rel err", err, flush=True)
+    assert err <= 1e-8
+    assert len(recs) == c.norb and all(abs(r["bp"][0] + r["bn"][0] - 1) < 1e-12 for r in recs)
+
+
+def test_green_exact_against_lehmann():
+    rng = np.random.default_rng(5)
+    norb, nbath, nu, nd = 2, 3, 2, 3
+    ns = norb + nbath
+    ek, vk = rng.uniform(-1.5, 1.5, ns), rng.uniform(0.2, 0.8, (norb, nbath))
+    U, J = 3.0, 0.4
+    om = omega_grid(64, 50.0)
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=1000.0, nev=1, nstep=200, sectors=[(nu, nd)],
+                  flags=MOED_F_EXACT_CMPLX) as s:
+        s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+    for iorb in (1, 2):
+        GL = np.zeros(len(om), complex)
+        for ispin in (1, 2):  # N_up != N_dn: both spins with factor 1/2
+            _, g = dense_ed.lehmann_green(norb, nbath, U, J, ek, vk, nu, nd, iorb - 1, ispin - 1, om)
+            GL += 0.5 * g
+        assert np.max(np.abs(G[iorb - 1] - GL)) <= 1e-9 * np.max(np.abs(GL))
+
+
+def test_green_cf_kernel_matches_oracle():
+    rng = np.random.default_rng(4)
+    n = 60
+    a = rng.uniform(-2, 2, n + 1)
+    b = rng.uniform(0.1, 1.0, n + 1)
+    om = omega_grid(300, 40.0)
+    with EDSolver(1, 1, 0.0, 0.0, [0.0, 0.0], [[0.5]]) as s:
+        for sign in (1, -1):
+            g = s.green_cf(sign, 0.37, n, a, b, om)
+            go = oracle.green_cf(sign, 0.37, n, a, b, om)
+            assert np.array_equal(g, go)  # same operations, no contraction: bit-identical
+
+
+def test_green_cf_adjoint_against_finite_differences():
+    rng = np.random.default_rng(8)
+    n = 20
+    a = rng.uniform(-2, 2, n + 1)
+    b = rng.uniform(0.1, 1.0, n + 1)
+    om = omega_grid(128, 30.0)
+    gbar = rng.standard_normal(len(om)) + 1j * rng.standard_normal(len(om))
+    with EDSolver(1, 1, 0.0, 0.0, [0.0, 0.0], [[0.5]], flags=MOED_F_EXACT_CMPLX) as s:
+        for sign in (1, -1):
+            ga, gb = s.green_cf_adjoint(sign, 0.2, n, a, b, om, gbar)
+
+            def L(aa, bb):
+                g = s.green_cf(sign, 0.2, n, aa, bb, om)
+                return np.sum(gbar.real * g.real + gbar.imag * g.imag)
+
+            h = 1e-6
+            for j in range(0, n + 1, 3):
+                ap, am = a.copy(), a.copy()
+                ap[j] += h
+                am[j] -= h
+                fd = (L(ap, b) - L(am, b)) / (2 * h)
+                assert abs(fd - ga[j]) <= 1e-6 * max(1.0, abs(fd))
+                bp, bm = b.copy(), b.copy()
+                bp[j] += h
+                bm[j] -= h
+                fd = (L(a, bp) - L(a, bm)) / (2 * h)
+                assert abs(fd - gb[j]) <= 1e-6 * max(1.0, abs(fd))
