@@ -78,11 +78,11 @@ def main():
     ap.add_argument("--config", default="ns14_roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--strip", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this solver runs one sector on one GPU; multi-GPU runs are not part of this build "
-              "(the sharded H v design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the H v benchmark runs one sector on one GPU; a run on more than one GPU is not "
+              "supported", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -92,13 +92,15 @@ def main():
     cfg = config(args.config)
     nu, nd = cfg.sectors[0]
     solver = moed.EDSolver.from_config(cfg)
-    if args.threads or args.rows:
-        solver.set_hv_shape(args.threads, args.rows)
+    if args.threads or args.strip:
+        solver.set_hv_shape(args.threads, args.strip)
     D = solver.dim(nu, nd)
     dev = torch.device("cuda:0")
     x = torch.empty(D, dtype=torch.float64, device=dev)
     y = torch.empty(D, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    # a dedicated (default-priority) stream: the kernels and the timing events share it
+    ts = torch.cuda.Stream(device=dev)
+    stream = ts.cuda_stream
     rc = moed.lib().moed_fill_start_device(solver.h, 12345, D, x.data_ptr(), stream)
     assert rc == 0
     for _ in range(args.warmup):
@@ -107,10 +109,10 @@ def main():
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
-    t0.record()
+    t0.record(ts)
     for _ in range(args.steps):
         solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
-    t1.record()
+    t1.record(ts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     ms = t0.elapsed_time(t1) / args.steps

@@ -124,8 +124,8 @@ int moed_green_record(moed_handle* h, int32_t irec, int32_t* iev, int32_t* iorb,
 int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
                   int32_t nw, const double* omega, double* g);
 /* adjoint of moed_green_cf: given gbar = dL/dRe g + i dL/dIm g (complex[nw]), returns
- * ga[j] = dL/da_j, gb[j] = dL/db_j for j = 0..nsc.  No reference counterpart (north-star
- * differentiation of G(iw) with respect to the Lanczos coefficients). */
+ * ga[j] = dL/da_j, gb[j] = dL/db_j for j = 0..nsc.  No reference counterpart: it
+ * differentiates G(iw) with respect to the Lanczos coefficients. */
 int moed_green_cf_adjoint(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a,
                           const double* b, int32_t nw, const double* omega, const double* gbar, double* ga,
                           double* gb);
@@ -149,8 +149,10 @@ int moed_omega_grid(int32_t nw, double beta, double* omega);
 /* counter-based start vector x_i = 2*u(splitmix64(seed ^ i)) - 1 on the device. */
 int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x_device, void* stream);
 
-/* launch shape of the H v kernel: threads per block and rows per block (0 = tuned/default). */
-int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t rows);
+/* launch shape of the H v kernels (no counterpart in the reference): row pass threads per block
+ * (0 = default 1024, else 256, 512 or 1024) and strip pass width in columns (0 = default 128,
+ * 64 or 128; 64 where the up-string count is odd).  Results are bit-identical for every shape. */
+int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip);
 /* device pointer of the handle's stream (hipStream_t). */
 void* moed_stream(moed_handle* h);
 

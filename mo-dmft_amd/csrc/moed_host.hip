@@ -65,13 +65,13 @@ inline int move_parity(uint32_t s, int p, int q) {
 }
 
 struct StrTable {
-    int n = 0, count = 0, maxhop = 0;
+    int n = 0, count = 0, mh = 0, nslot = 0, nq = 0;
     std::vector<uint32_t> str;
     std::vector<int32_t> rank;  // [2^Ns]: rank within this popcount class, -1 otherwise
-    DevBuf d_str, d_onsite, d_nhop, d_hop, d_imp;
+    DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec;
     StrDev dev() const {
-        return {d_str.as<uint16_t>(), d_onsite.as<double>(), d_nhop.as<uint8_t>(), d_hop.as<uint32_t>(),
-                d_imp.as<int32_t>(), count, maxhop};
+        return {d_str.as<uint16_t>(), d_nhop.as<int32_t>(), d_mv.as<uint32_t>(), d_mvc.as<double>(),
+                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    count, mh, nslot, nq};
     }
 };
 
@@ -81,6 +81,15 @@ struct SectorEig {
     int nfound = 0;
     std::vector<double> ev;  // nev values, DBL_MAX where none
     DevBuf vec;              // nfound vectors of D doubles
+};
+
+// One-body move slot of the H v row pass: bath slot (orbital bit p, bath bit q), applies when
+// exactly one of the two is occupied; impurity slot (p = i, q = j), applies when j is occupied and
+// i is empty (c+_i c_j).
+struct Slot {
+    int p, q;
+    bool imp;
+    double c;
 };
 
 struct GreenRecord {
@@ -113,13 +122,16 @@ struct moed_handle {
     std::vector<double> kept_val, kept_prob;
     std::vector<int> kept_sector, kept_level;
     std::vector<GreenRecord> records;
-    int hv_threads = 0, hv_rows = 0;
-    DevBuf ws[8];
+    std::vector<Slot> slots;
+    DevBuf d_slotc;
+    int hv_threads = 0, hv_strip = 0;
+    int ncu = 0;  // compute units of the device
+    DevBuf ws[9];
 };
 
 namespace {
 
-enum { WS_X = 0, WS_Y, WS_PART, WS_SCAL, WS_V, WS_W, WS_MISC, WS_MISC2 };
+enum { WS_X = 0, WS_Y, WS_PART, WS_SCAL, WS_V, WS_W, WS_MISC, WS_MISC2, WS_DOT };
 
 int fail(moed_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -152,7 +164,8 @@ int set_model(moed_handle* h, const double* ek, const double* vk) {
         if (!std::isfinite(v)) return fail(h, MOED_EARG, "ek contains a non-finite value");
     for (double v : h->vk)
         if (!std::isfinite(v)) return fail(h, MOED_EARG, "vk contains a non-finite value");
-    h->coef.assign((size_t)h->norb * h->nbath + (size_t)h->norb * h->norb, 0.0);
+    if ((size_t)h->norb * h->nbath + (size_t)h->norb * h->norb >= (size_t)kMaxCoef) return fail(h, MOED_EARG, "too many hopping coefficients");
+    h->coef.assign(kMaxCoef, 0.0);  // the last slot stays 0.0: padding entries of the move tables point at it
     for (size_t i = 0; i < h->vk.size(); ++i) h->coef[i] = h->vk[i];
     if (h->has_tij)
         for (int i = 0; i < h->norb * h->norb; ++i) h->coef[(size_t)h->norb * h->nbath + i] = h->tij[i];
@@ -162,6 +175,23 @@ int set_model(moed_handle* h, const double* ek, const double* vk) {
     HIPCHK(hipMemcpy(h->d_ek.p, ek16.data(), 16 * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(h->d_coef.ensure(h->coef.size() * sizeof(double)));
     HIPCHK(hipMemcpy(h->d_coef.p, h->coef.data(), h->coef.size() * sizeof(double), hipMemcpyHostToDevice));
+    // slots in the order of the move lists: (orbital, bath) pairs, then impurity pairs
+    h->slots.clear();
+    for (int o = 0; o < h->norb; ++o)
+        for (int b = 0; b < h->nbath; ++b)
+            if (h->coef[o + (size_t)h->norb * b] != 0.0)
+                h->slots.push_back({o, h->norb + b, false, h->coef[o + (size_t)h->norb * b]});
+    if (h->has_tij)
+        for (int i = 0; i < h->norb; ++i)
+            for (int j = 0; j < h->norb; ++j) {
+                const double t = h->coef[(size_t)h->norb * h->nbath + i + h->norb * j];
+                if (i != j && t != 0.0) h->slots.push_back({i, j, true, t});
+            }
+    if (h->slots.size() > 59) return fail(h, MOED_EARG, "too many one-body move slots");
+    std::vector<double> slotc(64, 0.0);  // the row pass reads up to 59 slots; the rest are 0.0
+    for (size_t k = 0; k < h->slots.size(); ++k) slotc[k] = h->slots[k].c;
+    HIPCHK(h->d_slotc.ensure(slotc.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(h->d_slotc.p, slotc.data(), slotc.size() * sizeof(double), hipMemcpyHostToDevice));
     h->tables.clear();  // onsite sums and move lists depend on ek, vk
     h->solved = false;
     return MOED_OK;
@@ -216,7 +246,7 @@ int build_table(moed_handle* h, int n, StrTable** out) {
                     }
                 }
         if ((int)mv.size() > kMaxHop) return fail(h, MOED_EARG, "too many one-body moves per string");
-        t->maxhop = std::max(t->maxhop, (int)mv.size());
+        t->mh = std::max(t->mh, (int)mv.size());
         for (int a = 0; a < norb; ++a)
             for (int b = 0; b < norb; ++b) {
                 if (a == b) continue;
@@ -226,24 +256,58 @@ int build_table(moed_handle* h, int n, StrTable** out) {
                 }
             }
     }
-    const int mh = std::max(t->maxhop, 1);
-    std::vector<uint32_t> hop((size_t)mh * C, 0u);
-    std::vector<uint8_t> nhop(C);
+    // row-major move lists, padded to a multiple of 8 (the strip pass reads them eight at a time)
+    const int mh = (std::max(t->mh, 1) + 7) / 8 * 8;
+    t->mh = mh;
+    std::vector<uint32_t> mvt((size_t)mh * C);
+    std::vector<double> mvc((size_t)mh * C, 0.0);
+    std::vector<int32_t> nhop(C);
     std::vector<uint16_t> str16(C);
     for (int r = 0; r < C; ++r) {
-        nhop[r] = (uint8_t)moves[r].size();
+        nhop[r] = (int32_t)moves[r].size();
         str16[r] = (uint16_t)t->str[r];
-        for (size_t k = 0; k < moves[r].size(); ++k) hop[k * C + r] = moves[r][k];
+        for (int k = 0; k < mh; ++k) {
+            if (k < nhop[r]) {
+                const uint32_t e = moves[r][k];
+                const double c = h->coef[(e >> 16) & 0xffu];
+                mvt[(size_t)r * mh + k] = e & 0xffffu;
+                mvc[(size_t)r * mh + k] = (e >> 31) ? -c : c;
+            } else {
+                mvt[(size_t)r * mh + k] = (uint32_t)r;
+            }
+        }
     }
-    HIPCHK(t->d_str.ensure(C * sizeof(uint16_t)));
-    HIPCHK(t->d_onsite.ensure(C * sizeof(double)));
-    HIPCHK(t->d_nhop.ensure(C));
-    HIPCHK(t->d_hop.ensure(hop.size() * sizeof(uint32_t)));
+    // row-pass records (moed_internal.h, StrDev::rec): onsite prefix, string, slot entries
+    t->nslot = (int)h->slots.size();
+    t->nq = (10 + 2 * t->nslot + 15) / 16;
+    const size_t ru16 = (size_t)t->nq * 8;  // uint16 per record
+    std::vector<uint16_t> rec(ru16 * C, (uint16_t)C);
+    for (int r = 0; r < C; ++r) {
+        const uint32_t s = t->str[r];
+        uint16_t* rr = rec.data() + ru16 * r;
+        std::memcpy(rr, &onsite[r], sizeof(double));
+        rr[4] = (uint16_t)s;
+        for (int k = 0; k < t->nslot; ++k) {
+            const Slot& sl = h->slots[k];
+            const uint32_t bp = (s >> sl.p) & 1u, bq = (s >> sl.q) & 1u;
+            const bool ok = sl.imp ? (bq && !bp) : (bp != bq);
+            if (!ok) continue;  // entry C: the zero
+            const uint32_t tgt = s ^ (1u << sl.p) ^ (1u << sl.q);
+            rr[5 + k] = (uint16_t)(move_parity(s, sl.p, sl.q) ? C + 1 + t->rank[tgt] : t->rank[tgt]);
+        }
+    }
+    HIPCHK(t->d_rec.ensure(rec.size() * sizeof(uint16_t)));
+    HIPCHK(hipMemcpy(t->d_rec.p, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    str16.resize(C + 2, 0);  // scalar reads fetch whole 4-byte words
+    HIPCHK(t->d_str.ensure((C + 2) * sizeof(uint16_t)));
+    HIPCHK(t->d_nhop.ensure(C * sizeof(int32_t)));
+    HIPCHK(t->d_mv.ensure(mvt.size() * sizeof(uint32_t)));
+    HIPCHK(t->d_mvc.ensure(mvc.size() * sizeof(double)));
     HIPCHK(t->d_imp.ensure(std::max<size_t>(imp.size(), 1) * sizeof(int32_t)));
-    HIPCHK(hipMemcpy(t->d_str.p, str16.data(), C * sizeof(uint16_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(t->d_onsite.p, onsite.data(), C * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(t->d_nhop.p, nhop.data(), C, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(t->d_hop.p, hop.data(), hop.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_str.p, str16.data(), (C + 2) * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_nhop.p, nhop.data(), C * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_mv.p, mvt.data(), mvt.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->d_mvc.p, mvc.data(), mvc.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!imp.empty()) HIPCHK(hipMemcpy(t->d_imp.p, imp.data(), imp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     *out = t.get();
     h->tables[n] = std::move(t);
@@ -270,21 +334,14 @@ int get_sector(moed_handle* h, int nu, int nd, Sector* s) {
     return MOED_OK;
 }
 
-void hv_shape(moed_handle* h, const Sector& s, int* threads, int* rows) {
-    int t = h->hv_threads, r = h->hv_rows;
-    if (t <= 0) t = 256;
-    if (r <= 0) r = 1;
-    while (r > 1 && (size_t)r * s.up->count * sizeof(double) > 64 * 1024) r >>= 1;
-    *threads = t;
-    *rows = r;
+HvPlan hv_plan_of(moed_handle* h, const Sector& s) {
+    return hv_plan(s.up->count, s.dn->count, h->hv_threads, h->hv_strip, h->ncu);
 }
 
-int hv_groups(moed_handle* h, const Sector& s) {
-    int t, r;
-    hv_shape(h, s, &t, &r);
-    return (s.dn->count + r - 1) / r;
-}
+// number of x.y partials hv_dev writes to dot_part
+int hv_groups(moed_handle* h, const Sector& s) { return hv_plan_of(h, s).grid_b; }
 
+// y = H x (x != y) on stream st; optional fused y *= 1/yscale[0] and x.y partials
 int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* dot_part, const double* yscale,
            const int32_t* stop, hipStream_t st) {
     HvArgs a{};
@@ -294,8 +351,7 @@ int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* 
     a.dn = s.dn->dev();
     a.ek = h->d_ek.as<double>();
     a.coef = h->d_coef.as<double>();
-    a.ncoef = (int)h->coef.size();
-    a.nsite = h->nsite;
+    a.slotc = h->d_slotc.as<double>();
     a.U = h->U;
     a.UmJ3 = h->UmJ3;
     a.Um2J = h->Um2J;
@@ -303,10 +359,7 @@ int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* 
     a.dot_part = dot_part;
     a.yscale = yscale;
     a.stop = stop;
-    int t, r;
-    hv_shape(h, s, &t, &r);
-    a.rows_per_block = r;
-    HIPCHK(launch_hv(a, h->norb, t, st));
+    HIPCHK(launch_hv(a, hv_plan_of(h, s), h->norb, st));
     return MOED_OK;
 }
 
@@ -344,7 +397,7 @@ int apply_c_dev(moed_handle* h, int pm, int iorb, int ispin, const Sector& in, c
 }
 
 int dot_dev(moed_handle* h, const double* a, const double* b, int64_t n, double* out_host, hipStream_t st) {
-    DevBuf& part = h->ws[WS_PART];
+    DevBuf& part = h->ws[WS_DOT];
     HIPCHK(part.ensure((kRedBlocks + 8) * sizeof(double)));
     double* p = part.as<double>();
     HIPCHK(launch_dot_partial(a, b, n, p, st));
@@ -744,6 +797,7 @@ int moed_create(const moed_params* p, moed_handle** out) {
     interaction_constants(h->U, h->J, &h->UmJ3, &h->Um2J);
     hipError_t e = hipSetDevice(h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e != hipSuccess) {
         delete h;
         return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
@@ -1106,13 +1160,13 @@ int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x, 
     return MOED_OK;
 }
 
-int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t rows) {
+int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip) {
     if (!h) return MOED_EARG;
     if (threads != 0 && threads != 256 && threads != 512 && threads != 1024)
         return fail(h, MOED_EARG, "threads must be 0, 256, 512 or 1024");
-    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(h, MOED_EARG, "rows must be 0, 1, 2 or 4");
+    if (strip != 0 && strip != 64 && strip != 128) return fail(h, MOED_EARG, "strip must be 0, 64 or 128");
     h->hv_threads = threads;
-    h->hv_rows = rows;
+    h->hv_strip = strip;
     return MOED_OK;
 }
 

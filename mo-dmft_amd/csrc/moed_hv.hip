@@ -1,0 +1,395 @@
+// The sector H v product (multiply_H, ed_hamiltonian.F90:88-191) for gfx950, in two passes.
+//
+// Vector layout in HBM: x[iu + nU*id] (up-string rank fastest, ed_basis.F90:120-123); a "row" is
+// the nU doubles of one down string id, a "column" the nD doubles of one up string iu.
+//   H = diag + T_up (x) 1 + 1 (x) T_dn + (spin flip, pair exchange)
+// T_up moves an up electron: it mixes the elements of one row; T_dn mixes the elements of one
+// column.  No single traversal order keeps both neighbourhoods on chip (a row's down-move
+// neighbours are rows all over the sector), so the product runs as
+//   row pass    y  = diag x + T_up x   one block stages one whole row of x in LDS and gathers
+//                                      the up moves there
+//   strip pass  y += T_dn x + SF + PE  a 128-column strip of x over all nD rows (nD * 1 KB,
+//                                      3.5 MB at 14 sites) is worked on by one XCD at a time, so
+//                                      every down-move read after the first hits that XCD's L2;
+//                                      this pass also applies 1/yscale and forms the x.y partials
+// HBM traffic per element: 16 bytes (row pass) + 24 bytes (strip pass: x, y in, y out).
+//
+// Every element's terms are summed in one fixed order, whatever the launch shape: the diagonal
+// product (rounded), then fused multiply-adds of the up-move slots in slot order, the down moves
+// in move-list order and spin flip / pair exchange per ordered impurity pair.  The diagonal itself is summed in the reference's order
+// (ed_hamiltonian.F90:110-139); terms a state does not have are added as +0.0, which leaves the
+// sum bit-identical (the sum starts at +0.0 and can never become -0.0).
+#include <hip/hip_runtime.h>
+
+#include "moed_device.h"
+#include "moed_internal.h"
+
+namespace moed {
+
+// ------------------------------------------------------------------ row pass
+
+// diagonal element of (up string su with onsite prefix Sup, down string sd), ed_hamiltonian.F90:110-139:
+// Sup, then ek of each occupied down site in ascending order (ekd[0..nd), in LDS), then the
+// interaction terms (ed_operators.F90:156-162) orbital by orbital and pair by pair.  fma(n, V, S)
+// with n in {0, 1} equals S + (n ? V : 0.0) exactly, and adding +0.0 for a term the state does
+// not have leaves the sum bit-identical (it starts at +0.0 and never becomes -0.0).
+template <int NORB>
+__device__ __forceinline__ double diag_element(double Sup, uint32_t su, uint32_t sd, const double* ekd, int nd,
+                                               double U, double UmJ3, double Um2J) {
+    double S = Sup;
+    for (int q = 0; q < nd; ++q) S += ekd[q];
+    double n[NORB];
+#pragma unroll
+    for (int o = 0; o < NORB; ++o) n[o] = (double)((su >> o) & 1u);
+#pragma unroll
+    for (int o = 0; o < NORB; ++o)
+        if ((sd >> o) & 1u) S = fma(n[o], U, S);
+#pragma unroll
+    for (int o = 0; o < NORB; ++o)
+#pragma unroll
+        for (int p = o + 1; p < NORB; ++p) {
+            S = fma(n[o] * n[p], UmJ3, S);
+            if ((sd >> p) & 1u) S = fma(n[o], Um2J, S);
+            if ((sd >> o) & 1u) S = fma(n[p], Um2J, S);
+            if ((sd >> o) & (sd >> p) & 1u) S += UmJ3;
+        }
+    return S;
+}
+
+// uint16 number i of a record held in registers
+template <int NW>
+__device__ __forceinline__ uint32_t rec_u16(const uint32_t (&w)[NW], int i) {
+    return (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xffffu);
+}
+
+// Row pass, one block per row (down string id).  The row is staged in LDS twice, as x and -x:
+//   entry k (0 <= k < nU): x[id][k];  entry nU: 0.0;  entry nU+1+k: -x[id][k]
+// A move with sign -1 reads the negated copy, so every slot term is one fma with the slot's
+// coefficient, and slots that do not apply (and the padding slots past nslot, coefficient 0.0)
+// read the zero: the KMAX = 2 NW - 5 slot reads carry no branches and are all in flight
+// together.  One column per thread and iteration; the column's record (onsite prefix, string,
+// slot entries) is read from global memory (L2-resident, shared by all rows).
+template <int NORB, int NW>
+__global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
+    constexpr int KMAX = 2 * NW - 5;
+    extern __shared__ double xs[];
+    __shared__ double ekd[kMaxSite];
+
+    if (P.stop && *P.stop) return;
+    const int NT = blockDim.x;
+    const int nU = P.up.count;
+    const int id = blockIdx.x;
+    const int neg = nU + 1;
+    const int tid = threadIdx.x;
+    const double* __restrict__ xr = P.x + (size_t)id * nU;
+    double* __restrict__ yr = P.y + (size_t)id * nU;
+
+    // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned)
+    if ((nU & 1) == 0) {
+        for (int p = tid; p < nU / 2; p += NT) {
+            const double2 v = reinterpret_cast<const double2*>(xr)[p];
+            reinterpret_cast<double2*>(xs)[p] = v;
+            xs[neg + 2 * p] = -v.x;
+            xs[neg + 2 * p + 1] = -v.y;
+        }
+    } else {
+        for (int k = tid; k < nU; k += NT) {
+            const double v = xr[k];
+            xs[k] = v;
+            xs[neg + k] = -v;
+        }
+    }
+    const uint32_t sd = scalar_u16(P.dn.str, id);
+    const int nd = __builtin_popcount(sd);
+    if (tid == 0) {
+        xs[nU] = 0.0;
+        xs[2 * nU + 1] = 0.0;
+        const cptr<double> ekc = as_const(P.ek);
+        uint32_t m = sd;
+        for (int q = 0; m; m &= m - 1, ++q) ekd[q] = ekc[__builtin_ctz(m)];
+    }
+    __syncthreads();
+
+    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
+    const cptr<double> slotc = as_const(P.slotc);
+    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(P.up.rec);
+    const int rstride = 4 * P.up.nq;
+    for (int iu = tid; iu < nU; iu += NT) {
+        uint32_t w[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) w[q] = recw[(size_t)iu * rstride + q];
+        const double Sup = __hiloint2double((int)w[1], (int)w[0]);
+        double acc = __dmul_rn(diag_element<NORB>(Sup, w[2] & 0xffffu, sd, ekd, nd, U, UmJ3, Um2J), xs[iu]);
+#pragma unroll
+        for (int q = 0; q < KMAX; ++q) acc = fma(slotc[q], xs[rec_u16(w, 5 + q)], acc);
+        yr[iu] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ strip pass
+
+// CPL consecutive columns of one row as one load (CPL = 2 needs an even nU: 16-byte aligned)
+template <int CPL>
+struct Cols {
+    double v[CPL];
+};
+template <int CPL>
+__device__ __forceinline__ Cols<CPL> load_cols(const double* p) {
+    Cols<CPL> c;
+    if constexpr (CPL == 2) {
+        const double2 t = *reinterpret_cast<const double2*>(p);
+        c.v[0] = t.x;
+        c.v[1] = t.y;
+    } else {
+        c.v[0] = *p;
+    }
+    return c;
+}
+
+// Work item (strip, row block): strip s = columns [W s, W s + W) with W = 64 CPL, row block b =
+// rows [rows_b b, rows_b (b + 1)).  Items are numbered strip-major and each XCD (blocks are dealt
+// to the 8 XCDs round-robin) takes a contiguous range, so the blocks resident on one XCD at any
+// time share one or two strips.  Each wave handles whole rows of its item's strip, CPL columns
+// per lane: the row's down moves and down impurity moves are wave-uniform scalar loads, and a
+// move that does not apply issues no load.
+template <int NORB, int CPL>
+__global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
+    constexpr int NW = kStripThreads / 64;
+    constexpr int NP = NORB * NORB;
+    constexpr int W = 64 * CPL;
+    __shared__ double s_red[NW];
+
+    if (P.stop && *P.stop) return;
+    const int nU = P.up.count, nD = P.dn.count;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = uniform_i32(tid >> 6);  // makes every per-row quantity below wave-uniform
+    const int nstrip = (nU + W - 1) / W;
+    const int bps = (nD + P.rows_b - 1) / P.rows_b;
+    const int item = (blockIdx.x % kXcd) * P.items_per_xcd + blockIdx.x / kXcd;
+    const double* __restrict__ X = P.x;
+    double* __restrict__ Y = P.y;
+    const cptr<uint32_t> mvp = as_const(P.dn.mv);
+    const cptr<double> mvcp = as_const(P.dn.mvc);
+    const cptr<int32_t> nhopc = as_const(P.dn.nhop);
+    const cptr<int32_t> dimpc = as_const(P.dn.imp);
+    const int mh = P.dn.mh;
+    const double J = P.J;
+    const double inv = P.yscale ? 1.0 / P.yscale[0] : 1.0;
+    double dsum = 0.0;
+
+    if (item < nstrip * bps) {
+        const int strip = item / bps, rb = item % bps;
+        const int c00 = strip * W + lane * CPL;
+        const bool ok = c00 < nU;          // with CPL = 2 and nU even, both columns are valid
+        const int iu = ok ? c00 : nU - CPL;  // idle lanes read valid columns and store nothing
+        const int r1 = min(nD, (rb + 1) * P.rows_b);
+        // up half of the spin-flip / pair-exchange terms: move j -> i of each column's up string
+        int32_t mu[NP > 1 ? NP : 1][CPL];
+        if constexpr (NORB > 1) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    mu[p][c] = (p / NORB != p % NORB) ? P.up.imp[(size_t)p * nU + iu + c] : -1;
+        }
+        for (int id = rb * P.rows_b + wave; id < r1; id += NW) {
+            const size_t ro = (size_t)id * nU;
+            // ---- scalar loads: the row's first eight down moves and its down impurity moves
+            const int nh = nhopc[id];
+            const size_t m0 = (size_t)id * mh;
+            uint32_t t[8];
+            double cf[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                t[q] = mvp[m0 + q];
+                cf[q] = mvcp[m0 + q];
+            }
+            int32_t msf[NP > 1 ? NP : 1], mpe[NP > 1 ? NP : 1];
+            if constexpr (NORB > 1) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    const int i = p / NORB, j = p % NORB;
+                    msf[p] = i != j ? dimpc[(size_t)(j * NORB + i) * nD + id] : -1;
+                    mpe[p] = i != j ? dimpc[(size_t)(i * NORB + j) * nD + id] : -1;
+                }
+            }
+            // ---- every vector load of the row before any use
+            Cols<CPL> acc = load_cols<CPL>(Y + ro + iu);
+            Cols<CPL> xo;
+            if (P.dot_part) xo = load_cols<CPL>(X + ro + iu);
+            Cols<CPL> xv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * nU + iu);
+            double gs[NP > 1 ? NP : 1][CPL], gp[NP > 1 ? NP : 1][CPL];
+            if constexpr (NORB > 1) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    if (p / NORB == p % NORB) continue;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        const int uu = mu[p][c] >= 0 ? (mu[p][c] & 0xffff) : iu + c;
+                        if (msf[p] >= 0) gs[p][c] = X[(size_t)(msf[p] & 0xffff) * nU + uu];
+                        if (mpe[p] >= 0) gp[p][c] = X[(size_t)(mpe[p] & 0xffff) * nU + uu];
+                    }
+                }
+            }
+            // ---- down-string moves in move order
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < nh) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc.v[c] = fma(cf[q], xv[q].v[c], acc.v[c]);
+                }
+            for (int k0 = 8; k0 < nh; k0 += 8) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    t[q] = mvp[m0 + k0 + q];
+                    cf[q] = mvcp[m0 + k0 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (k0 + q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * nU + iu);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (k0 + q < nh) {
+#pragma unroll
+                        for (int c = 0; c < CPL; ++c) acc.v[c] = fma(cf[q], xv[q].v[c], acc.v[c]);
+                    }
+            }
+            // ---- spin flip (i,j): up j->i, down i->j; pair exchange (i,j): up j->i, down j->i;
+            //      element J * sign_up * sign_dn (-J of ed_operators.F90:215,268 times the operator signs)
+            if constexpr (NORB > 1) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    if (p / NORB == p % NORB) continue;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        if (mu[p][c] >= 0) {
+                            const double cu = (mu[p][c] >> 16) ? -J : J;
+                            if (msf[p] >= 0) acc.v[c] = fma((msf[p] >> 16) ? -cu : cu, gs[p][c], acc.v[c]);
+                            if (mpe[p] >= 0) acc.v[c] = fma((mpe[p] >> 16) ? -cu : cu, gp[p][c], acc.v[c]);
+                        }
+                    }
+                }
+            }
+            if (P.yscale) {
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) acc.v[c] *= inv;
+            }
+            if (ok) {
+                if constexpr (CPL == 2) {
+                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
+                    __builtin_nontemporal_store(acc.v[1], Y + ro + iu + 1);
+                } else {
+                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
+                }
+                if (P.dot_part) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) dsum = fma(xo.v[c], acc.v[c], dsum);
+                }
+            }
+        }
+    }
+    if (P.dot_part) {
+        const double s = block_sum<kStripThreads>(dsum, s_red);
+        if (tid == 0) P.dot_part[blockIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------------ launch
+
+// record dwords the row pass reads: the smallest of 3, 5, 7, 9, 12, 16, 24, 32 with room for
+// the onsite prefix, the string and nslot slot entries (2 NW - 5 slots)
+int record_dwords(int nslot) {
+    const int need = (10 + 2 * nslot + 3) / 4;
+    for (int nw : {3, 5, 7, 9, 12, 16, 24, 32})
+        if (need <= nw) return nw;
+    return 0;
+}
+
+// threads_a: row pass threads per block (0 = 1024); strip: strip width 64 or 128 (0 = 128);
+// ncu: compute units of the device
+HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu) {
+    HvPlan p{};
+    p.threads_a = threads_a > 0 ? threads_a : 1024;
+    p.grid_a = nD;
+    (void)ncu;
+    if (strip <= 0) strip = 128;
+    p.cpl = (strip >= 128 && (nU % 2) == 0) ? 2 : 1;
+    p.rows_b = 4 * (kStripThreads / 64);
+    const int64_t w = 64 * p.cpl;
+    const int64_t nstrip = (nU + w - 1) / w;
+    const int64_t bps = (nD + p.rows_b - 1) / p.rows_b;
+    p.items_per_xcd = (int)((nstrip * bps + kXcd - 1) / kXcd);
+    p.grid_b = p.items_per_xcd * kXcd;
+    return p;
+}
+
+template <int NORB, int NW>
+static hipError_t rows_launch_t(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+    const size_t lds = (2 * (size_t)a.up.count + 2) * sizeof(double);
+    auto kern = hv_rows_kernel<NORB, NW>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(pl.grid_a), dim3(pl.threads_a), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int NORB>
+static hipError_t rows_launch(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+    switch (record_dwords(a.up.nslot)) {
+        case 3:
+            return rows_launch_t<NORB, 3>(a, pl, s);
+        case 5:
+            return rows_launch_t<NORB, 5>(a, pl, s);
+        case 7:
+            return rows_launch_t<NORB, 7>(a, pl, s);
+        case 9:
+            return rows_launch_t<NORB, 9>(a, pl, s);
+        case 12:
+            return rows_launch_t<NORB, 12>(a, pl, s);
+        case 16:
+            return rows_launch_t<NORB, 16>(a, pl, s);
+        case 24:
+            return rows_launch_t<NORB, 24>(a, pl, s);
+        case 32:
+            return rows_launch_t<NORB, 32>(a, pl, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+template <int NORB>
+static hipError_t hv_launch_norb(const HvArgs& a0, const HvPlan& pl, hipStream_t s) {
+    if (a0.up.count == 0 || a0.dn.count == 0) return hipSuccess;
+    HvArgs a = a0;
+    a.rows_b = pl.rows_b;
+    a.items_per_xcd = pl.items_per_xcd;
+    hipError_t e = rows_launch<NORB>(a, pl, s);
+    if (e != hipSuccess) return e;
+    if (pl.cpl == 2)
+        hipLaunchKernelGGL((hv_strip_kernel<NORB, 2>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hv(const HvArgs& a, const HvPlan& pl, int norb, hipStream_t s) {
+    switch (norb) {
+        case 1:
+            return hv_launch_norb<1>(a, pl, s);
+        case 2:
+            return hv_launch_norb<2>(a, pl, s);
+        case 3:
+            return hv_launch_norb<3>(a, pl, s);
+        case 4:
+            return hv_launch_norb<4>(a, pl, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace moed

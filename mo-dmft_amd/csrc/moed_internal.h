@@ -22,36 +22,61 @@ constexpr int kXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 // Structure-of-arrays over the string rank r (0-based, ascending integer order = the reference's
 // generate_basis order, ed_basis.F90:87-105):
 //   str[r]                 the bit string
-//   onsite[r]              sum of ek over the occupied sites in ascending site order, starting from
-//                          0.0 (the first partial sum of multiply_H's diagonal, ed_hamiltonian.F90:112-123)
 //   nhop[r]                number of one-body moves with a nonzero coefficient (V_ob, t_ij)
-//   hop[k*count + r]       move k: target rank (bits 0-15) | coefficient index (bits 16-23) | sign (bit 31)
+//   mv[r*mh + k], mvc[r*mh + k]
+//                          move k of string r (k < nhop[r]): target rank and signed coefficient;
+//                          rows padded to mh (a multiple of 8) with target r and coefficient 0.0
 //   imp[(a*norb+b)*count+r] impurity move b -> a (c+_a c_b), a != b: target rank | sign << 16, or -1
+//   rec[r*nq .. r*nq+nq)   row-pass record of string r, 16*nq bytes read as nq 16-byte loads:
+//                          uint16 [0,4) the onsite prefix (sum of ek over the occupied sites in
+//                          ascending site order from 0.0, the first partial sum of multiply_H's
+//                          diagonal, ed_hamiltonian.F90:112-123) as a double; [4] the string;
+//                          [5 + k] for slot k < nslot: the staged-row entry the move reads, rank
+//                          (sign +), count + 1 + rank (sign -, the negated copy), or count (zero)
+//                          when the slot does not apply.  A slot is a fixed (orbital, bath) pair or
+//                          an ordered impurity pair with coefficient slotc[k].
 struct StrDev {
     const uint16_t* str;
-    const double* onsite;
-    const uint8_t* nhop;
-    const uint32_t* hop;
+    const int32_t* nhop;
+    const uint32_t* mv;
+    const double* mvc;
     const int32_t* imp;
+    const uint4* rec;
     int32_t count;
-    int32_t maxhop;
+    int32_t mh;
+    int32_t nslot;
+    int32_t nq;
 };
 
+// Sector H v in two passes (moed_hv.hip).
+//   row pass:   y  = diag x + T_up x           (one block stages whole rows of x in LDS)
+//   strip pass: y += T_dn x + spin flip + pair exchange, then y *= 1/yscale and the x.y partials
+//               (a 64-column strip of x over all rows stays in one XCD's L2)
 struct HvArgs {
     const double* x;
     double* y;
     StrDev up, dn;
-    const double* ek;    // [nsite]
-    const double* coef;  // [ncoef]
-    int32_t ncoef;
-    int32_t nsite;
+    const double* ek;     // [nsite]
+    const double* coef;   // [kMaxCoef]: V_ob at o + norb*b, then t_ij; the last entry is 0.0
+    const double* slotc;  // [up.nslot] coefficients of the slots
     double U, UmJ3, Um2J, J;  // U, U-3J, U-2J, J exactly as the reference forms them (ed_operators.F90:156-162)
-    double* dot_part;         // optional: per-row partial of sum_i x_i y_i (nullptr = off)
+    double* dot_part;         // optional: per-block partial of sum_i x_i y_i (nullptr = off)
     const double* yscale;     // optional device scalar: y *= 1/yscale[0] (nullptr = off)
-    const int32_t* stop;      // optional device flag: kernel is a no-op when *stop != 0
-    int32_t rows_per_xcd;
-    int32_t rows_per_block;
+    const int32_t* stop;      // optional device flag: both passes are no-ops when *stop != 0
+    int32_t rows_b;           // strip pass: rows per block
+    int32_t items_per_xcd;    // strip pass: (strip, row block) items per XCD
 };
+
+constexpr int kStripThreads = 256;
+
+// launch geometry of the two passes for a sector with nU x nD elements
+struct HvPlan {
+    int threads_a, grid_a;              // row pass: one block per row
+    int cpl;                            // strip pass: columns per lane (strip width 64 * cpl)
+    int rows_b, items_per_xcd, grid_b;  // grid_b = number of x.y partials
+};
+HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu);
+hipError_t launch_hv(const HvArgs& a, const HvPlan& plan, int norb, hipStream_t s);
 
 // apply_c: per target string, the source rank and sign of the elementary operator
 struct ApplyCArgs {
@@ -63,7 +88,6 @@ struct ApplyCArgs {
 };
 
 // launchers (moed_kernels.hip)
-hipError_t launch_hv(const HvArgs& a, int norb, int threads, hipStream_t s);
 hipError_t launch_apply_c(const ApplyCArgs& a, hipStream_t s);
 hipError_t launch_fill_start(uint64_t seed, int64_t n, double* x, hipStream_t s);
 hipError_t launch_fill(double v, int64_t n, double* x, hipStream_t s);

@@ -6,32 +6,12 @@
 
 #include <cmath>
 
+#include "moed_device.h"
 #include "moed_internal.h"
 
 namespace moed {
 
 // ---------------------------------------------------------------- helpers
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// block sum in a fixed order (wave tree, then waves in index order); result valid in thread 0
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) {
-        for (int w = 0; w < NT / 64; ++w) s += red[w];
-    }
-    __syncthreads();
-    return s;
-}
 
 // Complex arithmetic as the reference build evaluates it: textbook division, a real operand
 // promoted to (x, 0), no contraction (every product and sum rounded on its own).
@@ -89,206 +69,6 @@ __device__ __forceinline__ double hypot_corrected(double x, double y) {
 }
 
 __device__ __forceinline__ double round_f32(double v) { return (double)(float)v; }
-
-// ---------------------------------------------------------------- H v
-
-// One block owns R consecutive down strings (rows).  The R input rows are staged in LDS; the
-// up-string moves of a column are read once from the column tables and applied to all R rows
-// from LDS; down-string moves are row-uniform and read whole rows of x (coalesced); the
-// spin-flip and pair-exchange terms gather x at (row', column').  The diagonal is summed per
-// element in the reference's order (ed_hamiltonian.F90:110-139): onsite up sites (precomputed
-// per column), onsite down sites, U, then the inter-orbital pairs in (uu, ud, du, dd) order.
-template <int NORB, int R, int NT>
-__global__ void __launch_bounds__(NT) hv_kernel(HvArgs P) {
-    extern __shared__ double xs[];  // R * nU staged input rows
-    __shared__ double s_coef[kMaxCoef];
-    __shared__ double s_ek[16];
-    __shared__ int32_t s_dt[R][kMaxHop];
-    __shared__ double s_dc[R][kMaxHop];
-    __shared__ int32_t s_ndh[R];
-    __shared__ uint32_t s_sd[R];
-    __shared__ int32_t s_dimp[R][NORB * NORB > 0 ? NORB * NORB : 1];
-    __shared__ double s_red[NT / 64];
-
-    if (P.stop && *P.stop) return;
-    const int nU = P.up.count, nD = P.dn.count;
-    const int b = blockIdx.x;
-    const int grp = (b % kXcd) * P.rows_per_xcd + b / kXcd;  // XCD k owns a contiguous band of rows
-    const int row0 = grp * R;
-    if (row0 >= nD) return;
-    const int nr = min(R, nD - row0);
-    const int tid = threadIdx.x;
-
-    for (int k = tid; k < P.ncoef; k += NT) s_coef[k] = P.coef[k];
-    if (tid < P.nsite) s_ek[tid] = P.ek[tid];
-    if (tid < R) {
-        const int d = row0 + tid;
-        s_sd[tid] = tid < nr ? P.dn.str[d] : 0u;
-        s_ndh[tid] = tid < nr ? (int)P.dn.nhop[d] : 0;
-    }
-    for (int k = tid; k < R * kMaxHop; k += NT) {
-        const int r = k / kMaxHop, q = k % kMaxHop, d = row0 + r;
-        if (r < nr && q < P.dn.maxhop) {
-            const uint32_t e = P.dn.hop[(size_t)q * nD + d];
-            double c = P.coef[(e >> 16) & 0xff];
-            s_dt[r][q] = (int32_t)(e & 0xffffu);
-            s_dc[r][q] = (e >> 31) ? -c : c;
-        }
-    }
-    if (NORB > 1) {
-        for (int k = tid; k < R * NORB * NORB; k += NT) {
-            const int r = k / (NORB * NORB), p = k % (NORB * NORB), d = row0 + r;
-            s_dimp[r][p] = r < nr ? P.dn.imp[(size_t)p * nD + d] : -1;
-        }
-    }
-    const double* xrow0 = P.x + (size_t)row0 * nU;
-    for (int k = tid; k < nr * nU; k += NT) xs[k] = xrow0[k];
-    __syncthreads();
-
-    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J, J = P.J;
-    const double inv = P.yscale ? 1.0 / P.yscale[0] : 1.0;
-    double dsum = 0.0;
-    for (int iu = tid; iu < nU; iu += NT) {
-        const uint32_t su = P.up.str[iu];
-        const double Sup = P.up.onsite[iu];
-        double acc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r < nr) {
-                const uint32_t sd = s_sd[r];
-                double S = Sup;
-                for (uint32_t m = sd; m; m &= m - 1) S += s_ek[__builtin_ctz(m)];
-#pragma unroll
-                for (int o = 0; o < NORB; ++o)
-                    if ((su >> o) & (sd >> o) & 1u) S += U;
-#pragma unroll
-                for (int o = 0; o < NORB; ++o)
-#pragma unroll
-                    for (int p = o + 1; p < NORB; ++p) {
-                        const uint32_t uo = (su >> o) & 1u, upp = (su >> p) & 1u;
-                        const uint32_t dO = (sd >> o) & 1u, dp = (sd >> p) & 1u;
-                        if (uo & upp) S += UmJ3;
-                        if (uo & dp) S += Um2J;
-                        if (dO & upp) S += Um2J;
-                        if (dO & dp) S += UmJ3;
-                    }
-                acc[r] = S * xs[r * nU + iu];
-            } else {
-                acc[r] = 0.0;
-            }
-        }
-        // up-string moves (hybridisation and impurity hopping of spin up)
-        const int nh = P.up.nhop[iu];
-        for (int k = 0; k < nh; ++k) {
-            const uint32_t e = P.up.hop[(size_t)k * nU + iu];
-            const int t = (int)(e & 0xffffu);
-            double c = s_coef[(e >> 16) & 0xff];
-            c = (e >> 31) ? -c : c;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (r < nr) acc[r] += c * xs[r * nU + t];
-        }
-        // down-string moves: whole rows
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r < nr) {
-                const int ndh = s_ndh[r];
-                for (int k = 0; k < ndh; ++k) acc[r] += s_dc[r][k] * P.x[(size_t)s_dt[r][k] * nU + iu];
-            }
-        }
-        // spin flip (i,j): up j->i, down i->j; pair exchange (i,j): up j->i, down j->i.
-        // Element J * sign_up * sign_dn (the -J of ed_operators.F90:215,268 times the operator signs).
-        if (NORB > 1) {
-#pragma unroll
-            for (int i = 0; i < NORB; ++i)
-#pragma unroll
-                for (int j = 0; j < NORB; ++j) {
-                    if (i == j) continue;
-                    const int32_t mu = P.up.imp[(size_t)(i * NORB + j) * nU + iu];
-                    if (mu < 0) continue;
-                    const int uu = mu & 0xffff;
-                    const double cu = (mu >> 16) ? -J : J;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        if (r >= nr) continue;
-                        const int32_t msf = s_dimp[r][j * NORB + i];
-                        if (msf >= 0) {
-                            const double c = (msf >> 16) ? -cu : cu;
-                            acc[r] += c * P.x[(size_t)(msf & 0xffff) * nU + uu];
-                        }
-                        const int32_t mpe = s_dimp[r][i * NORB + j];
-                        if (mpe >= 0) {
-                            const double c = (mpe >> 16) ? -cu : cu;
-                            acc[r] += c * P.x[(size_t)(mpe & 0xffff) * nU + uu];
-                        }
-                    }
-                }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r < nr) {
-                const double v = P.yscale ? acc[r] * inv : acc[r];
-                P.y[(size_t)(row0 + r) * nU + iu] = v;
-                if (P.dot_part) dsum += xs[r * nU + iu] * v;
-            }
-        }
-    }
-    if (P.dot_part) {
-        const double s = block_sum<NT>(dsum, s_red);
-        if (tid == 0) P.dot_part[grp] = s;
-    }
-}
-
-template <int NORB, int R, int NT>
-static hipError_t hv_launch_t(const HvArgs& a, hipStream_t s) {
-    const int ngrp = (a.dn.count + R - 1) / R;
-    HvArgs p = a;
-    p.rows_per_xcd = (ngrp + kXcd - 1) / kXcd;
-    p.rows_per_block = R;
-    const int grid = p.rows_per_xcd * kXcd;
-    const size_t lds = (size_t)R * a.up.count * sizeof(double);
-    hipLaunchKernelGGL((hv_kernel<NORB, R, NT>), dim3(grid), dim3(NT), lds, s, p);
-    return hipGetLastError();
-}
-
-template <int NORB, int R>
-static hipError_t hv_launch_nt(const HvArgs& a, int threads, hipStream_t s) {
-    switch (threads) {
-        case 512:
-            return hv_launch_t<NORB, R, 512>(a, s);
-        case 1024:
-            return hv_launch_t<NORB, R, 1024>(a, s);
-        default:
-            return hv_launch_t<NORB, R, 256>(a, s);
-    }
-}
-
-template <int NORB>
-static hipError_t hv_launch_r(const HvArgs& a, int threads, hipStream_t s) {
-    switch (a.rows_per_block) {
-        case 2:
-            return hv_launch_nt<NORB, 2>(a, threads, s);
-        case 4:
-            return hv_launch_nt<NORB, 4>(a, threads, s);
-        default:
-            return hv_launch_nt<NORB, 1>(a, threads, s);
-    }
-}
-
-hipError_t launch_hv(const HvArgs& a, int norb, int threads, hipStream_t s) {
-    switch (norb) {
-        case 1:
-            return hv_launch_r<1>(a, threads, s);
-        case 2:
-            return hv_launch_r<2>(a, threads, s);
-        case 3:
-            return hv_launch_r<3>(a, threads, s);
-        case 4:
-            return hv_launch_r<4>(a, threads, s);
-        default:
-            return hipErrorInvalidValue;
-    }
-}
 
 // ---------------------------------------------------------------- apply_c
 

@@ -318,8 +318,8 @@ class EDSolver:
                                         it.ctypes.data_as(_ip), st.ctypes.data_as(_ip)))
         return x, fret, it, st
 
-    def set_hv_shape(self, threads=0, rows=0):
-        self._check(lib().moed_set_hv_shape(self.h, threads, rows))
+    def set_hv_shape(self, threads=0, strip=0):
+        self._check(lib().moed_set_hv_shape(self.h, threads, strip))
 
     @property
     def stream(self):

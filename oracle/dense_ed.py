@@ -160,6 +160,8 @@ def hamiltonian(sector, norb, nbath, U, J, ek, vk, tij=None, mode=ref_mode):
         rows.append(sector.index(t[ok]))
         cols.append(src[ok])
         vals.append(c * sg[ok])
+    if not rows:
+        return sp.csr_matrix((sector.dim, sector.dim))
     r, cl, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals).astype(np.float64)
     return sp.csr_matrix((v, (r, cl)), shape=(sector.dim, sector.dim))
 

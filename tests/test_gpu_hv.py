@@ -91,9 +91,9 @@ def test_hv_product_8site_all_launch_shapes(configs):
     with EDSolver.from_config(c) as s:
         x = start_vector(int(z["x_seed"]), s.dim(nu, nd))
         ys = []
-        for threads in (256, 512, 1024):
-            for rows in (1, 2, 4):
-                s.set_hv_shape(threads, rows)
+        for threads in (0, 256, 512, 1024):
+            for strip in (0, 64, 128):
+                s.set_hv_shape(threads, strip)
                 ys.append(s.multiply_H(nu, nd, x))
     for y in ys[1:]:
         assert np.array_equal(y, ys[0])  # every element's terms are summed in the same order

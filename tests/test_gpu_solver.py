@@ -37,8 +37,9 @@ def test_lanczos_matches_reference_recurrence(configs):
     bb[0] = 1.0
     br = z["lz_b"].copy()
     br[0] = 1.0
-    g = oracle.green_cf(+1, -10.0, nsc, a, bb, om)
-    gr = oracle.green_cf(+1, -10.0, int(z["lz_nstep_calc"]), z["lz_a"], br, om)
+    # z = E + i w with E below the spectrum (E0 = -17.97), where the fraction is well conditioned
+    g = oracle.green_cf(+1, -30.0, nsc, a, bb, om)
+    gr = oracle.green_cf(+1, -30.0, int(z["lz_nstep_calc"]), z["lz_a"], br, om)
     assert np.max(np.abs(g - gr)) <= 1e-8 * np.max(np.abs(gr))
 
 
