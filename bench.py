@@ -70,6 +70,16 @@ def cpu_port_hv(cfg, nu, nd):
             "seconds_per_hv": sec}
 
 
+def hbm_traffic(path):
+    """HBM bytes per H v from the committed counter summary (line 'hbm_bytes_per_launch: N')"""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--strip", type=int, default=0)
+    ap.add_argument("--copy-check", type=int, default=0,
+                    help="after the timed region, N plain device copies of the same vector (a kernel of known "
+                         "traffic, 16*D bytes, that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the H v benchmark runs one sector on one GPU; a run on more than one GPU is not "
@@ -108,20 +121,42 @@ def main():
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
-    wall0 = time.perf_counter()
+    # eager: one library call per step
     t0.record(ts)
     for _ in range(args.steps):
         solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
     t1.record(ts)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = t0.elapsed_time(t1) / args.steps
+    ms_eager = t0.elapsed_time(t1) / args.steps
+    # the same K steps captured once in a HIP graph (no host launch gaps between the kernels)
+    ms_graph = None
+    try:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=ts):
+            for _ in range(args.steps):
+                solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
+        with torch.cuda.stream(ts):
+            g.replay()  # warm replay
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        with torch.cuda.stream(ts):
+            t0.record(ts)
+            g.replay()
+            t1.record(ts)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        ms_graph = t0.elapsed_time(t1) / args.steps
+    except Exception as e:  # capture unsupported: report the eager figure
+        print(f"bench.py: graph capture failed ({e}); timing the eager loop", file=sys.stderr)
+        wall = ms_eager * args.steps / 1e3
+    ms = ms_graph if ms_graph is not None else ms_eager
+    with torch.cuda.stream(ts):
+        for _ in range(args.copy_check):
+            y.copy_(x)
+    torch.cuda.synchronize()
     alg_bytes = 16.0 * D
     gbs = alg_bytes / (ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    traffic = hbm_traffic(os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.md"))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
@@ -134,9 +169,12 @@ def main():
                    "params": "seeded bath, tests/golden/configs.json"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "kernel": "hv_kernel"},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "kernel": "one H v = hv_rows_kernel + hv_strip_kernel on one stream"},
         "cpu_baseline": cpu,
         "wall_s_timed_region": round(wall, 4),
+        "ms_per_step_eager": round(ms_eager, 6),
+        "timing": "HIP graph of the K steps" if ms_graph is not None else "eager loop",
     }
     if cpu:
         out["gpu_over_cpu"] = round(gbs / cpu["value"], 1)
