@@ -70,6 +70,33 @@ def cpu_port_hv(cfg, nu, nd):
             "seconds_per_hv": sec}
 
 
+def batched_vs_plain(cfg_name="ns10_batched", reps=1):
+    """full solve (every sector, nev eigenpairs each, state selection) plus the Green's functions,
+    sector by sector and with all sectors in one multi-problem run; wall seconds of each"""
+    import moed
+    from moed.model import config, omega_grid
+    c = config(cfg_name)
+    om = omega_grid(c.nw, c.beta)
+    out = {}
+    for name, flags in (("plain", 0), ("batched", moed.MOED_F_BATCHED)):
+        best = None
+        for _ in range(reps + 1):  # the first run of each form also builds the string tables
+            s = moed.EDSolver.from_config(c, flags=flags)
+            t0 = time.perf_counter()
+            kept = s.ed_solve()
+            s.ed_calc_green_ftn(om)
+            t = time.perf_counter() - t0
+            s.close()
+            best = t if best is None else min(best, t)
+        out[name] = best
+        out[name + "_E0"] = kept[0][0]
+    return {"workload": f"{c.nsite}-site model, all {len(c.sectors)} sectors, {c.nev} eigenpairs each, "
+                        f"G(iw) on {c.nw} frequencies at beta={c.beta:g}",
+            "plain_s": round(out["plain"], 4), "batched_s": round(out["batched"], 4),
+            "speedup": round(out["plain"] / out["batched"], 2),
+            "E0_equal": bool(out["plain_E0"] == out["batched_E0"])}
+
+
 def hbm_traffic(path):
     """HBM bytes per H v from the committed counter summary (line 'hbm_bytes_per_launch: N')"""
     if not os.path.exists(path):
@@ -87,6 +114,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="ns14_roofline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--strip", type=int, default=0)
     ap.add_argument("--copy-check", type=int, default=0,
@@ -157,6 +185,7 @@ def main():
     alg_bytes = 16.0 * D
     gbs = alg_bytes / (ms * 1e-3) / 1e9
     traffic = hbm_traffic(os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.md"))
+    batched = None if args.no_batched else batched_vs_plain()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
@@ -172,6 +201,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes,
                      "kernel": "one H v = hv_rows_kernel + hv_strip_kernel on one stream"},
         "cpu_baseline": cpu,
+        "batched_vs_plain": batched,
         "wall_s_timed_region": round(wall, 4),
         "ms_per_step_eager": round(ms_eager, 6),
         "timing": "HIP graph of the K steps" if ms_graph is not None else "eager loop",

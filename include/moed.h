@@ -41,6 +41,8 @@ extern "C" {
 #define MOED_F_GF_REORTH 2u     /* full reorthogonalisation in the Green's-function Lanczos runs
                                    (the reference has none, ed_lanczos.F90:37-68) */
 #define MOED_F_NO_TUNING 4u     /* ignore the launch-shape tuning table */
+#define MOED_F_BATCHED 8u       /* moed_solve: all sectors in one multi-problem Lanczos run (one launch
+                                   per operation for every sector) instead of sector by sector */
 
 typedef struct moed_handle moed_handle;
 

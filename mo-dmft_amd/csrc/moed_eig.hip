@@ -1,0 +1,460 @@
+// Multi-problem engine: the kernels that let many independent problems (all the sectors of a
+// model, or many Lanczos runs) advance with one launch per operation.
+//
+//   hv_batch_kernel      single-pass sector H v, one block per (problem, row); for the small
+//                        sectors of a batch the whole vector of a problem sits in L2, so the
+//                        down-move rows are read directly.  Term order per element is that of
+//                        the two-pass product (diagonal, up slots, down moves, spin flip /
+//                        pair exchange), so both give bit-identical y.
+//   eig_* kernels        full-reorthogonalisation Lanczos (classical Gram-Schmidt with a DGKS
+//                        second pass) over packed vectors: per-item partial inner products,
+//                        per-problem reductions in a fixed order, projection, the step's
+//                        alpha / beta and the stop test, all on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "moed_device.h"
+#include "moed_hvops.h"
+#include "moed_internal.h"
+
+namespace moed {
+
+// ------------------------------------------------------------------ batched H v
+
+template <int NORB, int NW>
+__global__ void __launch_bounds__(kEigThreads) hv_batch_kernel(HvBatchArgs A) {
+    constexpr int KMAX = 2 * NW - 5;
+    constexpr int NP = NORB * NORB;
+    extern __shared__ double xs[];
+    __shared__ double ekd[kMaxSite];
+
+    const int item = blockIdx.x;
+    const int p = as_const(A.row_prob)[item];
+    if (A.skip && as_const(A.skip)[p]) return;
+    const int id = as_const(A.row_id)[item];
+    BatchProbDev d;  // scalar loads of the descriptor
+    {
+        static_assert(sizeof(BatchProbDev) % 8 == 0, "descriptor is read as 8-byte words");
+        const cptr<uint64_t> src = (cptr<uint64_t>)(const __attribute__((address_space(4))) void*)(A.prob + p);
+        uint64_t wds[sizeof(BatchProbDev) / 8];
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(BatchProbDev) / 8); ++q) wds[q] = src[q];
+        __builtin_memcpy(&d, wds, sizeof(d));
+    }
+    const int nU = d.up.count, nD = d.dn.count, neg = nU + 1;
+    const int tid = threadIdx.x;
+    const double* __restrict__ X = A.x + d.off;
+    double* __restrict__ Y = A.y + d.off;
+    const double* __restrict__ xr = X + (size_t)id * nU;
+
+    for (int k = tid; k < nU; k += kEigThreads) {
+        const double v = xr[k];
+        xs[k] = v;
+        xs[neg + k] = -v;
+    }
+    const uint32_t sd = scalar_u16(d.dn.str, id);
+    const int nd = __builtin_popcount(sd);
+    if (tid == 0) {
+        xs[nU] = 0.0;
+        xs[2 * nU + 1] = 0.0;
+        const cptr<double> ekc = as_const(A.ek);
+        uint32_t m = sd;
+        for (int q = 0; m; m &= m - 1, ++q) ekd[q] = ekc[__builtin_ctz(m)];
+    }
+    __syncthreads();
+
+    const double U = A.U, UmJ3 = A.UmJ3, Um2J = A.Um2J, J = A.J;
+    const cptr<double> slotc = as_const(A.slotc);
+    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(d.up.rec);
+    const int rstride = 4 * d.up.nq;
+    const cptr<uint32_t> mvp = as_const(d.dn.mv);
+    const cptr<double> mvcp = as_const(d.dn.mvc);
+    const int nh = as_const(d.dn.nhop)[id];
+    const size_t m0 = (size_t)id * d.dn.mh;
+    int32_t msf[NP > 1 ? NP : 1], mpe[NP > 1 ? NP : 1];
+    if constexpr (NORB > 1) {
+        const cptr<int32_t> dimpc = as_const(d.dn.imp);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const int i = q / NORB, j = q % NORB;
+            msf[q] = i != j ? dimpc[(size_t)(j * NORB + i) * nD + id] : -1;
+            mpe[q] = i != j ? dimpc[(size_t)(i * NORB + j) * nD + id] : -1;
+        }
+    }
+    const double inv = A.scale ? 1.0 / A.scale[(size_t)p * A.scale_stride] : 1.0;
+
+    for (int iu = tid; iu < nU; iu += kEigThreads) {
+        uint32_t w[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) w[q] = recw[(size_t)iu * rstride + q];
+        const double Sup = __hiloint2double((int)w[1], (int)w[0]);
+        double acc = __dmul_rn(diag_element<NORB>(Sup, w[2] & 0xffffu, sd, ekd, nd, U, UmJ3, Um2J), xs[iu]);
+#pragma unroll
+        for (int q = 0; q < KMAX; ++q) acc = fma(slotc[q], xs[rec_u16(w, 5 + q)], acc);
+        for (int k = 0; k < nh; ++k) acc = fma(mvcp[m0 + k], X[(size_t)mvp[m0 + k] * nU + iu], acc);
+        if constexpr (NORB > 1) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                if (q / NORB == q % NORB) continue;
+                const int32_t mu = d.up.imp[(size_t)q * nU + iu];
+                if (mu >= 0) {
+                    const int uu = mu & 0xffff;
+                    const double cu = (mu >> 16) ? -J : J;
+                    if (msf[q] >= 0) acc = fma((msf[q] >> 16) ? -cu : cu, X[(size_t)(msf[q] & 0xffff) * nU + uu], acc);
+                    if (mpe[q] >= 0) acc = fma((mpe[q] >> 16) ? -cu : cu, X[(size_t)(mpe[q] & 0xffff) * nU + uu], acc);
+                }
+            }
+        }
+        Y[(size_t)id * nU + iu] = A.scale ? acc * inv : acc;
+    }
+}
+
+template <int NORB, int NW>
+static hipError_t hv_batch_t(const HvBatchArgs& a, int nitems, int max_nU, hipStream_t s) {
+    const size_t lds = (2 * (size_t)max_nU + 2) * sizeof(double);
+    auto kern = hv_batch_kernel<NORB, NW>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nitems), dim3(kEigThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int NORB>
+static hipError_t hv_batch_nw(const HvBatchArgs& a, int nitems, int max_nU, int nslot, hipStream_t s) {
+    switch (record_dwords(nslot)) {
+        case 3:
+            return hv_batch_t<NORB, 3>(a, nitems, max_nU, s);
+        case 5:
+            return hv_batch_t<NORB, 5>(a, nitems, max_nU, s);
+        case 7:
+            return hv_batch_t<NORB, 7>(a, nitems, max_nU, s);
+        case 9:
+            return hv_batch_t<NORB, 9>(a, nitems, max_nU, s);
+        case 12:
+            return hv_batch_t<NORB, 12>(a, nitems, max_nU, s);
+        case 16:
+            return hv_batch_t<NORB, 16>(a, nitems, max_nU, s);
+        case 24:
+            return hv_batch_t<NORB, 24>(a, nitems, max_nU, s);
+        case 32:
+            return hv_batch_t<NORB, 32>(a, nitems, max_nU, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_hv_batch(const HvBatchArgs& a, int nitems, int max_nU, int norb, int nslot, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    switch (norb) {
+        case 1:
+            return hv_batch_nw<1>(a, nitems, max_nU, nslot, s);
+        case 2:
+            return hv_batch_nw<2>(a, nitems, max_nU, nslot, s);
+        case 3:
+            return hv_batch_nw<3>(a, nitems, max_nU, nslot, s);
+        case 4:
+            return hv_batch_nw<4>(a, nitems, max_nU, nslot, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ segmented vector kernels
+
+__device__ __forceinline__ bool eig_active(const EigArgs& e, int p, int pass) {
+    if (e.done[p]) return false;
+    return pass != 2 || e.pass2[p] != 0;
+}
+
+// part[item] = sum of a*b over the item (pass = 0: every problem)
+__global__ void __launch_bounds__(kEigThreads) eig_segdot_kernel(EigArgs e, const double* a, const double* b) {
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x;
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kEigChunk / kEigThreads; ++j) {
+        const int r = threadIdx.x + j * kEigThreads;
+        if (r < len) s = fma(a[base + r], b[base + r], s);
+    }
+    s = block_sum<kEigThreads>(s, red);
+    if (threadIdx.x == 0) e.part[item] = s;
+}
+
+// out[p * stride] = sum of part over p's items in item order (op 1: square root)
+__global__ void eig_segsum_kernel(EigArgs e, double* out, int stride, int op) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= e.P) return;
+    double s = 0.0;
+    const int i0 = e.pitem0[p], n = e.pnitem[p];
+    for (int i = 0; i < n; ++i) s += e.part[i0 + i];
+    out[(size_t)p * stride] = op == 1 ? sqrt(s) : s;
+}
+
+hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(eig_segdot_kernel, dim3(e.ni), dim3(kEigThreads), 0, s, e, a, b);
+    hipLaunchKernelGGL(eig_segsum_kernel, dim3((e.P + 63) / 64), dim3(64), 0, s, e, out, stride, op);
+    return hipGetLastError();
+}
+
+// part[i * ni + item] = <V_i, w> over the item for the VG basis vectors of group blockIdx.y
+// (i < k1), and <w, w> in slot k1 when with_self
+constexpr int kEigVG = 8;
+__global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, const double* V, const double* w,
+                                                                   int k1, int with_self, int pass) {
+    __shared__ double red[kEigVG][kEigThreads / 64];
+    const int item = blockIdx.x;
+    const int p = e.item_prob[item];
+    if (!eig_active(e, p, pass)) return;
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    constexpr int R = kEigChunk / kEigThreads;
+    double wv[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int r = threadIdx.x + j * kEigThreads;
+        wv[j] = r < len ? w[base + r] : 0.0;
+    }
+    const int g0 = blockIdx.y * kEigVG;
+    double acc[kEigVG];
+#pragma unroll
+    for (int q = 0; q < kEigVG; ++q) {
+        const int i = g0 + q;
+        double s = 0.0;
+        if (i < k1) {
+            const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int r = threadIdx.x + j * kEigThreads;
+                if (r < len) s = fma(u[r], wv[j], s);
+            }
+        } else if (i == k1 && with_self) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) s = fma(wv[j], wv[j], s);
+        }
+        acc[q] = wave_sum(s);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kEigVG; ++q) red[q][wid] = acc[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < kEigVG) {
+        const int i = g0 + threadIdx.x;
+        if (i < k1 + with_self) {
+            double s = 0.0;
+            for (int wv2 = 0; wv2 < kEigThreads / 64; ++wv2) s += red[threadIdx.x][wv2];
+            e.part[(size_t)i * e.ni + item] = s;
+        }
+    }
+}
+
+// h_i = sum of the item partials (ordered); c_i = h_i / nrm_i^2; alpha += h_k / nrm_k; |w|^2
+__global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, int k1, int with_self, int pass) {
+    const int p = blockIdx.x;
+    if (!eig_active(e, p, pass)) return;
+    const int i0 = e.pitem0[p], n = e.pnitem[p];
+    double* h = e.h + (size_t)p * e.ldh;
+    const double* nrm = e.nrm + (size_t)p * e.ldh;
+    for (int i = threadIdx.x; i < k1 + with_self; i += kEigThreads) {
+        double s = 0.0;
+        for (int it = 0; it < n; ++it) s += e.part[(size_t)i * e.ni + i0 + it];
+        h[i] = s;
+        if (i < k1) e.c[(size_t)p * e.ldh + i] = s / (nrm[i] * nrm[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* aux = e.aux + (size_t)p * 4;
+        aux[0] += h[k1 - 1] / nrm[k1 - 1];
+        if (with_self) aux[1] = h[k1];
+    }
+}
+
+// dst = src - sum_{i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item
+__global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, const double* V, const double* src,
+                                                                  double* dst, int k1, int pass) {
+    extern __shared__ double sc[];
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x;
+    const int p = e.item_prob[item];
+    if (!eig_active(e, p, pass)) return;
+    for (int i = threadIdx.x; i < k1; i += kEigThreads) sc[i] = e.c[(size_t)p * e.ldh + i];
+    __syncthreads();
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    constexpr int R = kEigChunk / kEigThreads;
+    double acc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int r = threadIdx.x + j * kEigThreads;
+        acc[j] = r < len ? src[base + r] : 0.0;
+    }
+    // (a full item: no bounds tests, eight basis vectors' loads in flight per step)
+    if (len == kEigChunk) {
+        int i = 0;
+        for (; i + 8 <= k1; i += 8) {
+            double u[8][R];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int j = 0; j < R; ++j) u[q][j] = V[(size_t)(i + q) * e.dtot + base + threadIdx.x + j * kEigThreads];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[j] = fma(-sc[i + q], u[q][j], acc[j]);
+        }
+        for (; i < k1; ++i)
+#pragma unroll
+            for (int j = 0; j < R; ++j) acc[j] = fma(-sc[i], V[(size_t)i * e.dtot + base + threadIdx.x + j * kEigThreads], acc[j]);
+    } else {
+        for (int i = 0; i < k1; ++i) {
+            const double ci = sc[i];
+            const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int r = threadIdx.x + j * kEigThreads;
+                if (r < len) acc[j] = fma(-ci, u[r], acc[j]);
+            }
+        }
+    }
+    double ss = 0.0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int r = threadIdx.x + j * kEigThreads;
+        if (r < len) {
+            dst[base + r] = acc[j];
+            ss = fma(acc[j], acc[j], ss);
+        }
+    }
+    ss = block_sum<kEigThreads>(ss, red);
+    if (threadIdx.x == 0) e.part[item] = ss;
+}
+
+// |w'|^2 from the item partials; DGKS test (pass 1); the step's alpha, beta and the stop test
+__global__ void eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= e.P || !eig_active(e, p, pass)) return;
+    const int i0 = e.pitem0[p], n = e.pnitem[p];
+    double wp2 = 0.0;
+    for (int it = 0; it < n; ++it) wp2 += e.part[i0 + it];
+    double* aux = e.aux + (size_t)p * 4;
+    aux[2] = wp2;
+    if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
+        e.pass2[p] = 1;
+        aux[1] = wp2;
+        return;
+    }
+    e.pass2[p] = 0;
+    double* alpha = e.alpha + (size_t)p * e.ldh;
+    double* beta = e.beta + (size_t)p * e.ldh;
+    alpha[k] = aux[0];
+    aux[0] = 0.0;
+    const double bk = sqrt(wp2);
+    double tnorm = 0.0;
+    for (int i = 0; i <= k; ++i) tnorm = fmax(tnorm, fabs(alpha[i]) + (i < k ? beta[i] : 0.0));
+    beta[k] = bk;
+    if (bk <= tiny * fmax(tnorm, 1e-300) || k + 1 >= e.mmax[p]) {
+        e.done[p] = 1;
+        e.mfinal[p] = k + 1;
+    } else {
+        e.nrm[(size_t)p * e.ldh + k + 1] = bk;
+    }
+}
+
+hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
+                           double tiny, hipStream_t s) {
+    const int k1 = k + 1;
+    const int with_self = pass == 1 ? 1 : 0;
+    const double* src = pass == 1 ? w : V + (size_t)k1 * e.dtot;
+    double* dst = const_cast<double*>(V) + (size_t)k1 * e.dtot;
+    const int ng = (k1 + with_self + kEigVG - 1) / kEigVG;
+    hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, k1, with_self, pass);
+    hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P), dim3(kEigThreads), 0, s, e, k1, with_self, pass);
+    hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * sizeof(double), s, e, V, src,
+                       dst, k1, pass);
+    hipLaunchKernelGGL(eig_finish_kernel, dim3((e.P + 63) / 64), dim3(64), 0, s, e, k, pass, eta2, tiny);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Ritz vectors
+
+// y = sum_{i < m_p} S_i V_i over the item, written to out[p] + j*dim_p; part[j*ni + item] = |y|^2
+__global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const double* V, const double* S, int lds,
+                                                               int nev, const int32_t* nf, double* const* out) {
+    extern __shared__ double ss[];
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x, j = blockIdx.y;
+    const int p = e.item_prob[item];
+    if (j >= nf[p]) return;
+    const int m = e.mfinal[p];
+    const double* Sp = S + ((size_t)p * nev + j) * lds;
+    for (int i = threadIdx.x; i < m; i += kEigThreads) ss[i] = Sp[i];
+    __syncthreads();
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    constexpr int R = kEigChunk / kEigThreads;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double si = ss[i];
+        const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int t = threadIdx.x + r * kEigThreads;
+            if (t < len) acc[r] = fma(si, u[t], acc[r]);
+        }
+    }
+    double* y = out[p] + (size_t)j * e.pdim[p] + (base - e.poff[p]);
+    double s2 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int t = threadIdx.x + r * kEigThreads;
+        if (t < len) {
+            y[t] = acc[r];
+            s2 = fma(acc[r], acc[r], s2);
+        }
+    }
+    s2 = block_sum<kEigThreads>(s2, red);
+    if (threadIdx.x == 0) e.part[(size_t)j * e.ni + item] = s2;
+}
+
+__global__ void eig_ritz_norm_kernel(EigArgs e, int nev, const int32_t* nf, double* n2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e.P * nev) return;
+    const int p = t / nev, j = t % nev;
+    if (j >= nf[p]) return;
+    const int i0 = e.pitem0[p], n = e.pnitem[p];
+    double s = 0.0;
+    for (int it = 0; it < n; ++it) s += e.part[(size_t)j * e.ni + i0 + it];
+    n2[t] = s;
+}
+
+__global__ void __launch_bounds__(kEigThreads) eig_ritz_scale_kernel(EigArgs e, int nev, const int32_t* nf,
+                                                                     double* const* out, const double* n2) {
+    const int item = blockIdx.x, j = blockIdx.y;
+    const int p = e.item_prob[item];
+    if (j >= nf[p]) return;
+    const double f = 1.0 / sqrt(n2[(size_t)p * nev + j]);
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    double* y = out[p] + (size_t)j * e.pdim[p] + (base - e.poff[p]);
+    for (int t = threadIdx.x; t < len; t += kEigThreads) y[t] = y[t] * f;
+}
+
+hipError_t launch_eig_ritz(const EigArgs& e, const double* V, const double* S, int lds, int nev,
+                           const int32_t* nf, double* const* out, double* n2, hipStream_t s) {
+    hipLaunchKernelGGL(eig_ritz_kernel, dim3(e.ni, nev), dim3(kEigThreads), (size_t)lds * sizeof(double), s, e, V, S,
+                       lds, nev, nf, out);
+    hipLaunchKernelGGL(eig_ritz_norm_kernel, dim3((e.P * nev + 63) / 64), dim3(64), 0, s, e, nev, nf, n2);
+    hipLaunchKernelGGL(eig_ritz_scale_kernel, dim3(e.ni, nev), dim3(kEigThreads), 0, s, e, nev, nf, out, n2);
+    return hipGetLastError();
+}
+
+}  // namespace moed

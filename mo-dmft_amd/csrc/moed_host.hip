@@ -12,6 +12,8 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 #include "moed_fit.h"
 #include "moed_internal.h"
@@ -127,11 +129,29 @@ struct moed_handle {
     int hv_threads = 0, hv_strip = 0;
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
+    DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
 };
 
 namespace {
 
 enum { WS_X = 0, WS_Y, WS_PART, WS_SCAL, WS_V, WS_W, WS_MISC, WS_MISC2, WS_DOT };
+
+// f(0..n-1) on up to 16 host threads (independent work items)
+template <class F>
+void parallel_for(int n, F&& f) {
+    const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (int i = next++; i < n; i = next++) f(i);
+        });
+    for (auto& t : th) t.join();
+}
 
 int fail(moed_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -497,117 +517,249 @@ int eig_dense(moed_handle* h, const Sector& s, int nev, SectorEig& E) {
     return MOED_OK;
 }
 
-// Lanczos with full reorthogonalisation (classical Gram-Schmidt with a DGKS second pass),
-// stopping when the lowest nev Ritz pairs have relative residual |beta_m z_{m,i}| <= tol |theta_i|
-// or the Krylov space is exhausted; Ritz vectors formed as V S on the device.
-int eig_lanczos(moed_handle* h, const Sector& s, int nev, uint64_t seed, SectorEig& E) {
+// Lanczos with full reorthogonalisation (classical Gram-Schmidt with a DGKS second pass) for
+// several sectors at once; one sector is the per-sector form of the same code.  Every problem p
+// runs the same iteration as alone: start vector u_0 (counter hash, seed of its sector), then per
+// step w = H u_k / |u_k|, u_{k+1} = w - sum_i <u_i, w>/|u_i|^2 u_i (second pass when |u_{k+1}|^2 <
+// |w|^2 / 2), alpha_k, beta_k = |u_{k+1}|.  A problem stops when the Krylov space is exhausted
+// (beta_k <= 1e-14 |T|), at its basis cap, or when the lowest nev Ritz pairs have relative
+// residual |beta_m z_{m,i}| <= tol |theta_i| (tested on the host after steps m < 20 and every
+// fifth step after, with one synchronisation for all problems).  Ritz vectors V S on the device.
+// The vectors are packed end to end; every launch serves all problems (moed_eig.hip).
+enum { EG_V = 0, EG_W, EG_I32, EG_I64, EG_DBL, EG_DESC, EG_OUT, EG_N };
+
+int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out) {
     hipStream_t st = h->stream;
-    const int64_t D = s.D;
-    const int mmax = (int)std::min<int64_t>(h->gs_maxvec, D);
-    DevBuf& V = h->ws[WS_V];
-    DevBuf& W = h->ws[WS_W];
-    DevBuf& part = h->ws[WS_PART];
-    DevBuf& sc = h->ws[WS_SCAL];
-    const int nblk = multidot_blocks(D);
-    HIPCHK(V.ensure((size_t)(mmax + 1) * D * sizeof(double)));
-    HIPCHK(W.ensure((size_t)D * sizeof(double)));
-    HIPCHK(part.ensure((size_t)(mmax + 2) * nblk * sizeof(double) + 64 * sizeof(double)));
-    HIPCHK(sc.ensure((size_t)(4 * (mmax + 2)) * sizeof(double)));
-    double* Vp = V.as<double>();
-    double* w = W.as<double>();
-    double* pp = part.as<double>();
-    double* d_h = sc.as<double>();            // [mmax+2]
-    double* d_c = d_h + (mmax + 2);           // [mmax+2]
-    double* d_nrm = d_c + (mmax + 2);         // [mmax+2]
-    double* d_misc = d_nrm + (mmax + 2);      // scalars
-    std::vector<double> nrm(mmax + 2), alpha, beta, hh(mmax + 2), cc(mmax + 2);
-    HIPCHK(launch_fill_start(seed, D, Vp, st));
-    double n0;
-    int rc = dot_dev(h, Vp, Vp, D, &n0, st);
-    if (rc) return rc;
-    nrm[0] = std::sqrt(n0);
-    HIPCHK(hipMemcpyAsync(d_nrm, nrm.data(), sizeof(double), hipMemcpyHostToDevice, st));
-    const double eta2 = 0.5;  // DGKS: second pass when |w'|^2 < |w|^2 / 2
-    int m = 0;
-    bool done = false;
-    std::vector<double> ritz_z;  // last row of Z at the last check
-    while (!done) {
-        const int k = m;
-        double* uk = Vp + (size_t)k * D;
-        rc = hv_dev(h, s, uk, w, nullptr, d_nrm + k, nullptr, st);  // w = H v_k
+    const int P = (int)secs.size();
+    if (P == 0) return MOED_OK;
+    std::vector<Sector> S(P);
+    std::vector<int64_t> off(P), dim(P);
+    std::vector<int32_t> mmax(P);
+    int64_t dtot = 0;
+    int M = 1;
+    for (int p = 0; p < P; ++p) {
+        int rc = get_sector(h, secs[p].first, secs[p].second, &S[p]);
         if (rc) return rc;
-        double alpha_k = 0.0, wnorm2 = 0.0, wp2 = 0.0;
-        for (int pass = 0; pass < 2; ++pass) {
-            HIPCHK(launch_multidot(Vp, D, k + 1, pass == 0 ? w : Vp + (size_t)(k + 1) * D, D, pp, nblk, st));
-            HIPCHK(launch_multidot_final(pp, k + 1, nblk, d_h, st));
-            HIPCHK(hipMemcpyAsync(hh.data(), d_h, (k + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+        dim[p] = S[p].D;
+        off[p] = dtot;
+        dtot += dim[p];
+        mmax[p] = (int32_t)std::min<int64_t>(h->gs_maxvec, dim[p]);
+        M = std::max(M, (int)mmax[p]);
+    }
+    const int ldh = M + 2;
+    // vector items and H v row items (sectors up to 2^20 elements: one batched launch; larger
+    // sectors: the two-pass product, one launch pair per sector)
+    constexpr int64_t kSmall = (int64_t)1 << 20;
+    std::vector<int32_t> item_prob, item_len, pitem0(P), pnitem(P), row_prob, row_id;
+    std::vector<int64_t> item_start;
+    std::vector<BatchProbDev> desc(P);
+    int max_nU = 1;
+    for (int p = 0; p < P; ++p) {
+        pitem0[p] = (int32_t)item_prob.size();
+        for (int64_t s0 = 0; s0 < dim[p]; s0 += kEigChunk) {
+            item_prob.push_back(p);
+            item_start.push_back(off[p] + s0);
+            item_len.push_back((int32_t)std::min<int64_t>(kEigChunk, dim[p] - s0));
+        }
+        pnitem[p] = (int32_t)item_prob.size() - pitem0[p];
+        desc[p] = {S[p].up->dev(), S[p].dn->dev(), off[p]};
+        if (dim[p] <= kSmall) {
+            for (int id = 0; id < S[p].dn->count; ++id) {
+                row_prob.push_back(p);
+                row_id.push_back(id);
+            }
+            max_nU = std::max(max_nU, S[p].up->count);
+        }
+    }
+    const int ni = (int)item_prob.size(), nr = (int)row_prob.size();
+    // device memory (handle-owned, grown as needed)
+    DevBuf* g = h->eng;
+    const size_t npart = std::max<size_t>((size_t)ldh * ni, (size_t)nev * ni);
+    HIPCHK(g[EG_V].ensure((size_t)(M + 1) * dtot * sizeof(double)));
+    HIPCHK(g[EG_W].ensure((size_t)dtot * sizeof(double)));
+    const size_t n32 = 2 * (size_t)ni + 7 * (size_t)P + 2 * (size_t)nr;
+    HIPCHK(g[EG_I32].ensure(n32 * sizeof(int32_t)));
+    HIPCHK(g[EG_I64].ensure(((size_t)ni + 2 * P) * sizeof(int64_t)));
+    const size_t ndbl = 5 * (size_t)P * ldh + 4 * (size_t)P + npart + (size_t)P * nev * ldh + (size_t)P * nev;
+    HIPCHK(g[EG_DBL].ensure(ndbl * sizeof(double)));
+    HIPCHK(g[EG_DESC].ensure((size_t)P * sizeof(BatchProbDev)));
+    HIPCHK(g[EG_OUT].ensure((size_t)P * sizeof(double*)));
+    int32_t* i32 = g[EG_I32].as<int32_t>();
+    int32_t *d_item_prob = i32, *d_item_len = d_item_prob + ni, *d_pitem0 = d_item_len + ni, *d_pnitem = d_pitem0 + P,
+            *d_mmax = d_pnitem + P, *d_done = d_mmax + P, *d_pass2 = d_done + P, *d_mfinal = d_pass2 + P,
+            *d_nf = d_mfinal + P, *d_row_prob = d_nf + P, *d_row_id = d_row_prob + nr;
+    int64_t *d_item_start = g[EG_I64].as<int64_t>(), *d_poff = d_item_start + ni, *d_pdim = d_poff + P;
+    double* dbl = g[EG_DBL].as<double>();
+    double *d_alpha = dbl, *d_beta = d_alpha + (size_t)P * ldh, *d_nrm = d_beta + (size_t)P * ldh,
+           *d_h = d_nrm + (size_t)P * ldh, *d_c = d_h + (size_t)P * ldh, *d_aux = d_c + (size_t)P * ldh,
+           *d_part = d_aux + 4 * (size_t)P, *d_S = d_part + npart, *d_n2 = d_S + (size_t)P * nev * ldh;
+    std::vector<int32_t> zeroP(P, 0);
+    HIPCHK(hipMemcpyAsync(d_item_prob, item_prob.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_item_len, item_len.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pitem0, pitem0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pnitem, pnitem.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_mmax, mmax.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_done, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pass2, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_mfinal, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (nr > 0) {
+        HIPCHK(hipMemcpyAsync(d_row_prob, row_prob.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_row_id, row_id.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d_item_start, item_start.data(), ni * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_poff, off.data(), P * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pdim, dim.data(), P * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(g[EG_DESC].p, desc.data(), P * sizeof(BatchProbDev), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_fill(0.0, (int64_t)(5 * (size_t)P * ldh + 4 * (size_t)P), dbl, st));
+
+    EigArgs e{};
+    e.P = P;
+    e.ni = ni;
+    e.dtot = dtot;
+    e.ldh = ldh;
+    e.item_prob = d_item_prob;
+    e.item_start = d_item_start;
+    e.item_len = d_item_len;
+    e.pitem0 = d_pitem0;
+    e.pnitem = d_pnitem;
+    e.poff = d_poff;
+    e.pdim = d_pdim;
+    e.mmax = d_mmax;
+    e.done = d_done;
+    e.pass2 = d_pass2;
+    e.mfinal = d_mfinal;
+    e.alpha = d_alpha;
+    e.beta = d_beta;
+    e.nrm = d_nrm;
+    e.h = d_h;
+    e.c = d_c;
+    e.aux = d_aux;
+    e.part = d_part;
+
+    double* V = g[EG_V].as<double>();
+    double* W = g[EG_W].as<double>();
+    for (int p = 0; p < P; ++p)
+        HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p], V + off[p], st));
+    HIPCHK(launch_eig_segdot(e, V, V, d_nrm, ldh, 1, st));  // nrm[p][0] = |u_0|
+
+    HvBatchArgs hb{};
+    hb.prob = g[EG_DESC].as<BatchProbDev>();
+    hb.row_prob = d_row_prob;
+    hb.row_id = d_row_id;
+    hb.y = W;
+    hb.ek = h->d_ek.as<double>();
+    hb.slotc = h->d_slotc.as<double>();
+    hb.U = h->U;
+    hb.UmJ3 = h->UmJ3;
+    hb.Um2J = h->Um2J;
+    hb.J = h->J;
+    hb.scale_stride = ldh;
+    hb.skip = d_done;
+
+    std::vector<int32_t> done_h(P, 0), mfin_h(P, 0);
+    std::vector<double> al_h((size_t)P * ldh), be_h((size_t)P * ldh);
+    int active = P;
+    for (int k = 0; k < M && active > 0; ++k) {
+        const double* Vk = V + (size_t)k * dtot;
+        hb.x = Vk;
+        hb.scale = d_nrm + k;
+        HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
+        for (int p = 0; p < P; ++p) {
+            if (dim[p] <= kSmall || done_h[p]) continue;
+            int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
+            if (rc) return rc;
+        }
+        HIPCHK(launch_eig_pass(e, V, W, k, 1, 0.5, 1e-14, st));
+        HIPCHK(launch_eig_pass(e, V, W, k, 2, 0.5, 1e-14, st));
+        const int m = k + 1;
+        if (m < 20 || m % 5 == 0 || m == M) {
+            HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(mfin_h.data(), d_mfinal, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(al_h.data(), d_alpha, al_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(be_h.data(), d_beta, be_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            for (int i = 0; i <= k; ++i) cc[i] = hh[i] / (nrm[i] * nrm[i]);
-            alpha_k += hh[k] / nrm[k];
-            if (pass == 0) wnorm2 = hh[k + 1];
-            HIPCHK(hipMemcpyAsync(d_c, cc.data(), (k + 1) * sizeof(double), hipMemcpyHostToDevice, st));
-            HIPCHK(launch_project_out(Vp, D, k + 1, d_c, pass == 0 ? w : Vp + (size_t)(k + 1) * D,
-                                      Vp + (size_t)(k + 1) * D, D, pp, nblk, st));
-            HIPCHK(launch_reduce_final(pp, nblk, d_misc, 0, 0, nullptr, st));
-            HIPCHK(hipMemcpyAsync(&wp2, d_misc, sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (wp2 >= eta2 * wnorm2) break;
-            wnorm2 = wp2;
-        }
-        alpha.push_back(alpha_k);
-        const double bk = std::sqrt(wp2);
-        m = k + 1;
-        // convergence check on T_m
-        double tnorm = 0.0;
-        for (int i = 0; i < m; ++i) tnorm = std::max(tnorm, std::fabs(alpha[i]) + (i < (int)beta.size() ? beta[i] : 0.0));
-        const bool exhausted = bk <= 1e-14 * std::max(tnorm, 1e-300) || m >= mmax;
-        bool check = exhausted || (m >= nev && (m % 5 == 0 || m < 20));
-        if (check) {
-            std::vector<double> d(alpha), e(beta);
-            std::vector<double> Z((size_t)m, 0.0);
-            Z[m - 1] = 1.0;  // the last row of the identity
-            if (!tridiag_ql(d, e, 1, &Z)) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
-            const int nc = std::min(nev, m);
-            bool conv = m >= nev || exhausted;
-            for (int i = 0; i < nc && conv; ++i)
-                if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) conv = false;
-            if (conv || exhausted) done = true;
-        }
-        if (!done) {
-            beta.push_back(bk);
-            nrm[m] = bk;
-            HIPCHK(hipMemcpyAsync(d_nrm + m, &nrm[m], sizeof(double), hipMemcpyHostToDevice, st));
+            // convergence of the lowest nev Ritz pairs, every active problem in parallel
+            std::vector<int> conv(P, 0), qlbad(P, 0);
+            parallel_for(P, [&](int p) {
+                if (done_h[p] || m < nev) return;
+                std::vector<double> d(al_h.begin() + (size_t)p * ldh, al_h.begin() + (size_t)p * ldh + m);
+                std::vector<double> ee(be_h.begin() + (size_t)p * ldh, be_h.begin() + (size_t)p * ldh + (m - 1));
+                std::vector<double> Z((size_t)m, 0.0);
+                Z[m - 1] = 1.0;
+                if (!tridiag_ql(d, ee, 1, &Z)) {
+                    qlbad[p] = 1;
+                    return;
+                }
+                const double bk = be_h[(size_t)p * ldh + m - 1];
+                bool ok = true;
+                for (int i = 0; i < std::min(nev, m) && ok; ++i)
+                    if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) ok = false;
+                conv[p] = ok;
+            });
+            bool changed = false;
+            active = 0;
+            for (int p = 0; p < P; ++p) {
+                if (qlbad[p]) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
+                if (done_h[p]) continue;
+                if (conv[p]) {
+                    done_h[p] = 1;
+                    mfin_h[p] = m;
+                    changed = true;
+                    continue;
+                }
+                ++active;
+            }
+            if (changed) {
+                HIPCHK(hipMemcpyAsync(d_done, done_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(d_mfinal, mfin_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            }
         }
     }
-    // Ritz vectors from the full eigenvector matrix of T_m
-    std::vector<double> d(alpha), e(beta);
-    std::vector<double> Z((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
-    if (!tridiag_ql(d, e, m, &Z)) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
-    const int nf = std::min(nev, m);
-    E.nfound = nf;
-    E.ev.assign(nev, DBL_MAX);
-    for (int i = 0; i < nf; ++i) E.ev[i] = d[i];
-    std::vector<double> S((size_t)m * nf);
-    for (int i = 0; i < nf; ++i)
-        for (int k = 0; k < m; ++k) S[k + (size_t)m * i] = Z[(size_t)k * m + i] / nrm[k];
-    HIPCHK(E.vec.ensure((size_t)nf * D * sizeof(double)));
-    DevBuf& Sb = h->ws[WS_MISC];
-    HIPCHK(Sb.ensure(S.size() * sizeof(double)));
-    HIPCHK(hipMemcpyAsync(Sb.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    for (int i0 = 0; i0 < nf; i0 += 16) {
-        const int cnt = std::min(16, nf - i0);
-        HIPCHK(launch_ritz(Vp, D, m, Sb.as<double>() + (size_t)m * i0, cnt, E.vec.as<double>() + (size_t)i0 * D, D, D,
-                           st));
+    // final Ritz pairs: T_m of every problem on the host (in parallel), S = Z / |u_k|
+    HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(mfin_h.data(), d_mfinal, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(al_h.data(), d_alpha, al_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(be_h.data(), d_beta, be_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    std::vector<double> nrm_h((size_t)P * ldh);
+    HIPCHK(hipMemcpyAsync(nrm_h.data(), d_nrm, nrm_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> S_h((size_t)P * nev * ldh, 0.0);
+    std::vector<int32_t> nf(P, 0);
+    std::vector<int> ok(P, 1);
+    auto ritz = [&](int p) {
+        const int m = mfin_h[p];
+        std::vector<double> d(al_h.begin() + (size_t)p * ldh, al_h.begin() + (size_t)p * ldh + m);
+        std::vector<double> ee(be_h.begin() + (size_t)p * ldh, be_h.begin() + (size_t)p * ldh + (m - 1));
+        std::vector<double> Z((size_t)m * m, 0.0);
+        for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+        if (!tridiag_ql(d, ee, m, &Z)) {
+            ok[p] = 0;
+            return;
+        }
+        nf[p] = std::min(nev, m);
+        out[p]->ev.assign(nev, DBL_MAX);
+        for (int i = 0; i < nf[p]; ++i) {
+            out[p]->ev[i] = d[i];
+            for (int kk = 0; kk < m; ++kk)
+                S_h[((size_t)p * nev + i) * ldh + kk] = Z[(size_t)kk * m + i] / nrm_h[(size_t)p * ldh + kk];
+        }
+    };
+    parallel_for(P, ritz);
+    for (int p = 0; p < P; ++p)
+        if (!ok[p]) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
+    std::vector<double*> outp(P);
+    for (int p = 0; p < P; ++p) {
+        out[p]->nu = secs[p].first;
+        out[p]->nd = secs[p].second;
+        out[p]->D = dim[p];
+        out[p]->nfound = nf[p];
+        HIPCHK(out[p]->vec.ensure((size_t)std::max(nf[p], 1) * dim[p] * sizeof(double)));
+        outp[p] = out[p]->vec.as<double>();
     }
-    // normalise
-    for (int i = 0; i < nf; ++i) {
-        double* y = E.vec.as<double>() + (size_t)i * D;
-        double n2;
-        rc = dot_dev(h, y, y, D, &n2, st);
-        if (rc) return rc;
-        HIPCHK(launch_scale_copy(y, 1.0 / std::sqrt(n2), D, y, st));
-    }
+    HIPCHK(hipMemcpyAsync(d_S, S_h.data(), S_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_nf, nf.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(g[EG_OUT].p, outp.data(), P * sizeof(double*), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_eig_ritz(e, V, d_S, ldh, nev, d_nf, g[EG_OUT].as<double*>(), d_n2, st));
     HIPCHK(hipStreamSynchronize(st));
     return MOED_OK;
 }
@@ -620,7 +772,8 @@ int eig_sector(moed_handle* h, int nu, int nd, int nev, SectorEig& E) {
     E.nd = nd;
     E.D = s.D;
     if (s.D <= 64) return eig_dense(h, s, nev, E);
-    return eig_lanczos(h, s, nev, 0x5eed0000ull + (uint64_t)nu * 131 + nd, E);
+    std::vector<SectorEig*> out{&E};
+    return eig_batch(h, {{nu, nd}}, nev, out);
 }
 
 // SORT of func.f90:301-353: heapsort returning a 1-based index, with the same sift order so that
@@ -938,14 +1091,35 @@ int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int
     h->eig.clear();
     h->solved = false;
     std::vector<double> all((size_t)nsec * nev);
-    for (int is = 0; is < nsec; ++is) {
-        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
-        auto E = std::make_unique<SectorEig>();
-        int rc = eig_sector(h, nu, nd, nev, *E);
+    for (int is = 0; is < nsec; ++is) h->eig.push_back(std::make_unique<SectorEig>());
+    if (h->flags & MOED_F_BATCHED) {
+        // every sector above the dense size in one multi-problem Lanczos run
+        std::vector<std::pair<int, int>> big;
+        std::vector<SectorEig*> bout;
+        for (int is = 0; is < nsec; ++is) {
+            const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+            Sector s;
+            int rc = get_sector(h, nu, nd, &s);
+            if (rc) return rc;
+            if (s.D <= 64) {
+                rc = eig_sector(h, nu, nd, nev, *h->eig[is]);
+                if (rc) return rc;
+            } else {
+                big.push_back({nu, nd});
+                bout.push_back(h->eig[is].get());
+            }
+        }
+        int rc = eig_batch(h, big, nev, bout);
         if (rc) return rc;
-        for (int i = 0; i < nev; ++i) all[(size_t)is * nev + i] = E->ev[i];
-        h->eig.push_back(std::move(E));
+    } else {
+        for (int is = 0; is < nsec; ++is) {
+            const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+            int rc = eig_sector(h, nu, nd, nev, *h->eig[is]);
+            if (rc) return rc;
+        }
     }
+    for (int is = 0; is < nsec; ++is)
+        for (int i = 0; i < nev; ++i) all[(size_t)is * nev + i] = h->eig[is]->ev[i];
     // ed_solver.F90:65-90
     std::vector<int> ind(all.size());
     heapsort_index((int)all.size(), all.data(), ind.data());

@@ -22,45 +22,12 @@
 #include <hip/hip_runtime.h>
 
 #include "moed_device.h"
+#include "moed_hvops.h"
 #include "moed_internal.h"
 
 namespace moed {
 
 // ------------------------------------------------------------------ row pass
-
-// diagonal element of (up string su with onsite prefix Sup, down string sd), ed_hamiltonian.F90:110-139:
-// Sup, then ek of each occupied down site in ascending order (ekd[0..nd), in LDS), then the
-// interaction terms (ed_operators.F90:156-162) orbital by orbital and pair by pair.  fma(n, V, S)
-// with n in {0, 1} equals S + (n ? V : 0.0) exactly, and adding +0.0 for a term the state does
-// not have leaves the sum bit-identical (it starts at +0.0 and never becomes -0.0).
-template <int NORB>
-__device__ __forceinline__ double diag_element(double Sup, uint32_t su, uint32_t sd, const double* ekd, int nd,
-                                               double U, double UmJ3, double Um2J) {
-    double S = Sup;
-    for (int q = 0; q < nd; ++q) S += ekd[q];
-    double n[NORB];
-#pragma unroll
-    for (int o = 0; o < NORB; ++o) n[o] = (double)((su >> o) & 1u);
-#pragma unroll
-    for (int o = 0; o < NORB; ++o)
-        if ((sd >> o) & 1u) S = fma(n[o], U, S);
-#pragma unroll
-    for (int o = 0; o < NORB; ++o)
-#pragma unroll
-        for (int p = o + 1; p < NORB; ++p) {
-            S = fma(n[o] * n[p], UmJ3, S);
-            if ((sd >> p) & 1u) S = fma(n[o], Um2J, S);
-            if ((sd >> o) & 1u) S = fma(n[p], Um2J, S);
-            if ((sd >> o) & (sd >> p) & 1u) S += UmJ3;
-        }
-    return S;
-}
-
-// uint16 number i of a record held in registers
-template <int NW>
-__device__ __forceinline__ uint32_t rec_u16(const uint32_t (&w)[NW], int i) {
-    return (i & 1) ? (w[i >> 1] >> 16) : (w[i >> 1] & 0xffffu);
-}
 
 // Row pass, one block per row (down string id).  The row is staged in LDS twice, as x and -x:
 //   entry k (0 <= k < nU): x[id][k];  entry nU: 0.0;  entry nU+1+k: -x[id][k]

@@ -77,6 +77,75 @@ struct HvPlan {
 };
 HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu);
 hipError_t launch_hv(const HvArgs& a, const HvPlan& plan, int norb, hipStream_t s);
+// record dwords the row pass reads for nslot move slots (3, 5, 7, 9, 12, 16, 24 or 32)
+int record_dwords(int nslot);
+
+// ------------------------------------------------------------------ multi-problem engine (moed_eig.hip)
+//
+// Several independent problems (sectors, or Lanczos runs in different sectors) advance together:
+// their vectors are packed end to end (problem p at offset off[p], length dim[p]) and one launch
+// serves all of them.  Vector work is split into items of kEigChunk elements that never straddle
+// two problems; the H v of small sectors is one item per (problem, row).
+
+constexpr int kEigChunk = 1024;   // elements per vector item (256 threads x 4)
+constexpr int kEigThreads = 256;
+
+struct BatchProbDev {
+    StrDev up, dn;
+    int64_t off;
+};
+
+// single-pass H v over the rows of many small sectors: y_p = H_p x_p (/ scale_p)
+struct HvBatchArgs {
+    const BatchProbDev* prob;
+    const int32_t* row_prob;   // [nrow items]
+    const int32_t* row_id;     // [nrow items] down-string rank of the item
+    const double* x;           // packed
+    double* y;                 // packed
+    const double* ek;
+    const double* slotc;
+    double U, UmJ3, Um2J, J;
+    const double* scale;       // optional: y_p *= 1 / scale[p * scale_stride]
+    int32_t scale_stride;
+    const int32_t* skip;       // optional: problem p is a no-op when skip[p] != 0
+};
+hipError_t launch_hv_batch(const HvBatchArgs& a, int nitems, int max_nU, int norb, int nslot, hipStream_t s);
+
+// vector items and per-problem bookkeeping of the engine
+struct EigArgs {
+    int32_t P, ni;
+    int64_t dtot;              // packed length (stride between basis vectors)
+    int32_t ldh;               // row length of the per-problem scalar tables (>= M + 2)
+    const int32_t* item_prob;  // [ni]
+    const int64_t* item_start; // [ni] packed index of the item's first element
+    const int32_t* item_len;   // [ni]
+    const int32_t* pitem0;     // [P] first item of problem p
+    const int32_t* pnitem;     // [P]
+    const int64_t* poff;       // [P] packed offset of problem p
+    const int64_t* pdim;       // [P] its length
+    const int32_t* mmax;       // [P] basis size cap
+    int32_t* done;             // [P] 1 when the problem has stopped
+    int32_t* pass2;            // [P] DGKS second pass requested
+    int32_t* mfinal;           // [P] basis size at stop
+    double* alpha;             // [P][ldh]
+    double* beta;              // [P][ldh]
+    double* nrm;               // [P][ldh] norms of the unnormalised basis vectors
+    double* h;                 // [P][ldh] inner products of the current pass
+    double* c;                 // [P][ldh] projection coefficients
+    double* aux;               // [P][4]: alpha accumulator, |w|^2, |w'|^2
+    double* part;              // [(M + 2) * ni] per-item partial sums
+};
+// part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
+hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
+                             hipStream_t s);
+// one reorthogonalisation pass at step k (basis V_0..V_k, packed with stride dtot):
+//   pass 1: h_i = <V_i, w>, |w|^2; V_{k+1} = w - sum_i c_i V_i; DGKS test
+//   pass 2: (problems flagged) the same with w = V_{k+1}, in place; then the step's alpha, beta
+hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
+                           double tiny, hipStream_t s);
+// Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
+hipError_t launch_eig_ritz(const EigArgs& e, const double* V, const double* S, int lds, int nev,
+                           const int32_t* nf, double* const* out, double* n2, hipStream_t s);
 
 // apply_c: per target string, the source rank and sign of the elementary operator
 struct ApplyCArgs {

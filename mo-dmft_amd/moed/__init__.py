@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(HERE, "libmoed.so")
 MOED_F_EXACT_CMPLX = 1
 MOED_F_GF_REORTH = 2
 MOED_F_NO_TUNING = 4
+MOED_F_BATCHED = 8
 
 _ERR = {-1: "EARG", -2: "ESECTOR", -3: "EZERO", -4: "EHIP", -5: "ENOMEM", -6: "ESTATE", -7: "ECONV"}
 

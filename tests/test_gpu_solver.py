@@ -8,6 +8,7 @@ import pytest
 
 import oracle
 from oracle import dense_ed
+import moed
 from moed import EDSolver, MOED_F_EXACT_CMPLX
 from moed.model import load_configs, omega_grid, start_vector
 
@@ -89,16 +90,18 @@ def test_solve_ground_state_12site_against_arpack(configs):
     assert abs(ev[0] - E0) <= 1e-10 * abs(E0)
 
 
-def test_solve_all_sectors_small_model_and_selection():
+@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
+def test_solve_all_sectors_small_model_and_selection(flags):
     """every sector of a 5-site model (dims 1..100): eigenvalues vs dense ED, state selection
-    vs the oracle's restatement of ed_solve (SORT + boltzmann_factor)"""
+    vs the oracle's restatement of ed_solve (SORT + boltzmann_factor); sector by sector and with
+    all sectors in one multi-problem run"""
     rng = np.random.default_rng(21)
     norb, nbath = 2, 3
     ns = norb + nbath
     ek, vk = rng.uniform(-1.5, 1.5, ns), rng.uniform(0.2, 0.8, (norb, nbath))
     U, J, beta, nev = 3.0, 0.4, 5.0, 4
     sectors = [(a, b) for a in range(ns + 1) for b in range(ns + 1)]
-    with EDSolver(norb, nbath, U, J, ek, vk, beta=beta, nev=nev, nstep=30, sectors=sectors) as s:
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=beta, nev=nev, nstep=30, sectors=sectors, flags=flags) as s:
         kept = s.ed_solve()
         evs = [s.sector_eigenvalues(i + 1) for i in range(len(sectors))]
     allv = []
@@ -113,6 +116,35 @@ def test_solve_all_sectors_small_model_and_selection():
     assert len(kept) == len(ref)
     for k, r in zip(kept, ref):
         assert k[2:] == r[2:] and abs(k[0] - r[0]) < 1e-10 and abs(k[1] - r[1]) < 1e-9
+
+
+def test_batched_solve_and_green_equal_sector_by_sector():
+    """7-site model, all 64 sectors (dims up to 1225): the multi-problem run gives the same
+    eigenvalues, kept states and G(iw) as the sector-by-sector run, bit for bit (every problem
+    follows the same arithmetic alone or in the batch)"""
+    rng = np.random.default_rng(77)
+    norb, nbath = 2, 5
+    ns = norb + nbath
+    ek = rng.uniform(-2, 2, ns)
+    vk = np.zeros((norb, nbath))
+    for b in range(nbath):
+        vk[b % norb, b] = rng.uniform(0.2, 0.9)
+    sectors = [(a, b) for a in range(ns + 1) for b in range(ns + 1)]
+    om = moed.omega_grid(256, 20.0)
+    out = []
+    for flags in (0, moed.MOED_F_BATCHED):
+        with EDSolver(norb, nbath, 3.0, 0.5, ek, vk, beta=20.0, nev=4, nstep=40, sectors=sectors,
+                      flags=flags) as s:
+            kept = s.ed_solve()
+            evs = np.concatenate([s.sector_eigenvalues(i + 1) for i in range(len(sectors))])
+            out.append((kept, evs, s.ed_calc_green_ftn(om)))
+    (k0, e0, g0), (k1, e1, g1) = out
+    assert k0 == k1
+    assert np.array_equal(e0, e1)
+    assert np.array_equal(g0, g1)
+    w = np.linalg.eigvalsh(dense_ed.hamiltonian(dense_ed.Sector(ns, 3, 4), norb, nbath, 3.0, 0.5, ek, vk).toarray())
+    isec = sectors.index((3, 4))
+    assert np.allclose(e0[4 * isec:4 * isec + 4], w[:4], rtol=0, atol=1e-10 * abs(w[0]))
 
 
 def test_green_8site_against_oracle(configs):
