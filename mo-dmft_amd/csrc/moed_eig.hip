@@ -169,6 +169,15 @@ __device__ __forceinline__ bool eig_active(const EigArgs& e, int p, int pass) {
     return pass != 2 || e.pass2[p] != 0;
 }
 
+// sum of part[row * ni + i0 + it] over it < n by one wave: lane l adds items l, l + 64, ... in order,
+// then the lanes are combined in a fixed tree (same result for every launch)
+__device__ __forceinline__ double wave_item_sum(const double* part, size_t row0, int n) {
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int it = lane; it < n; it += 64) s += part[row0 + it];
+    return wave_sum(s);
+}
+
 // part[item] = sum of a*b over the item (pass = 0: every problem)
 __global__ void __launch_bounds__(kEigThreads) eig_segdot_kernel(EigArgs e, const double* a, const double* b) {
     __shared__ double red[kEigThreads / 64];
@@ -185,20 +194,17 @@ __global__ void __launch_bounds__(kEigThreads) eig_segdot_kernel(EigArgs e, cons
     if (threadIdx.x == 0) e.part[item] = s;
 }
 
-// out[p * stride] = sum of part over p's items in item order (op 1: square root)
-__global__ void eig_segsum_kernel(EigArgs e, double* out, int stride, int op) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= e.P) return;
-    double s = 0.0;
-    const int i0 = e.pitem0[p], n = e.pnitem[p];
-    for (int i = 0; i < n; ++i) s += e.part[i0 + i];
-    out[(size_t)p * stride] = op == 1 ? sqrt(s) : s;
+// out[p * stride] = sum of part over p's items (op 1: square root); one wave per problem
+__global__ void __launch_bounds__(64) eig_segsum_kernel(EigArgs e, double* out, int stride, int op) {
+    const int p = blockIdx.x;
+    const double s = wave_item_sum(e.part, e.pitem0[p], e.pnitem[p]);
+    if (threadIdx.x == 0) out[(size_t)p * stride] = op == 1 ? sqrt(s) : s;
 }
 
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
                              hipStream_t s) {
     hipLaunchKernelGGL(eig_segdot_kernel, dim3(e.ni), dim3(kEigThreads), 0, s, e, a, b);
-    hipLaunchKernelGGL(eig_segsum_kernel, dim3((e.P + 63) / 64), dim3(64), 0, s, e, out, stride, op);
+    hipLaunchKernelGGL(eig_segsum_kernel, dim3(e.P), dim3(64), 0, s, e, out, stride, op);
     return hipGetLastError();
 }
 
@@ -255,25 +261,21 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
     }
 }
 
-// h_i = sum of the item partials (ordered); c_i = h_i / nrm_i^2; alpha += h_k / nrm_k; |w|^2
+// h_i = sum of the item partials (one wave per vector i); c_i = h_i / nrm_i^2; the wave of vector
+// k adds h_k / nrm_k to the step's alpha, the wave of slot k1 stores |w|^2
 __global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, int k1, int with_self, int pass) {
     const int p = blockIdx.x;
     if (!eig_active(e, p, pass)) return;
-    const int i0 = e.pitem0[p], n = e.pnitem[p];
-    double* h = e.h + (size_t)p * e.ldh;
+    const int i = blockIdx.y * (kEigThreads / 64) + (threadIdx.x >> 6);
+    if (i >= k1 + with_self) return;
+    const double sum = wave_item_sum(e.part, (size_t)i * e.ni + e.pitem0[p], e.pnitem[p]);
+    if ((threadIdx.x & 63) != 0) return;
     const double* nrm = e.nrm + (size_t)p * e.ldh;
-    for (int i = threadIdx.x; i < k1 + with_self; i += kEigThreads) {
-        double s = 0.0;
-        for (int it = 0; it < n; ++it) s += e.part[(size_t)i * e.ni + i0 + it];
-        h[i] = s;
-        if (i < k1) e.c[(size_t)p * e.ldh + i] = s / (nrm[i] * nrm[i]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double* aux = e.aux + (size_t)p * 4;
-        aux[0] += h[k1 - 1] / nrm[k1 - 1];
-        if (with_self) aux[1] = h[k1];
-    }
+    double* aux = e.aux + (size_t)p * 4;
+    e.h[(size_t)p * e.ldh + i] = sum;
+    if (i < k1) e.c[(size_t)p * e.ldh + i] = sum / (nrm[i] * nrm[i]);
+    if (i == k1 - 1) aux[0] += sum / nrm[i];
+    if (i == k1) aux[1] = sum;
 }
 
 // dst = src - sum_{i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item
@@ -337,12 +339,11 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
 }
 
 // |w'|^2 from the item partials; DGKS test (pass 1); the step's alpha, beta and the stop test
-__global__ void eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= e.P || !eig_active(e, p, pass)) return;
-    const int i0 = e.pitem0[p], n = e.pnitem[p];
-    double wp2 = 0.0;
-    for (int it = 0; it < n; ++it) wp2 += e.part[i0 + it];
+__global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
+    const int p = blockIdx.x;
+    if (!eig_active(e, p, pass)) return;
+    const double wp2 = wave_item_sum(e.part, e.pitem0[p], e.pnitem[p]);
+    if (threadIdx.x != 0) return;
     double* aux = e.aux + (size_t)p * 4;
     aux[2] = wp2;
     if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
@@ -375,10 +376,11 @@ hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, i
     double* dst = const_cast<double*>(V) + (size_t)k1 * e.dtot;
     const int ng = (k1 + with_self + kEigVG - 1) / kEigVG;
     hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, k1, with_self, pass);
-    hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P), dim3(kEigThreads), 0, s, e, k1, with_self, pass);
+    hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P, (k1 + with_self + 3) / 4), dim3(kEigThreads), 0, s, e, k1,
+                       with_self, pass);
     hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * sizeof(double), s, e, V, src,
                        dst, k1, pass);
-    hipLaunchKernelGGL(eig_finish_kernel, dim3((e.P + 63) / 64), dim3(64), 0, s, e, k, pass, eta2, tiny);
+    hipLaunchKernelGGL(eig_finish_kernel, dim3(e.P), dim3(64), 0, s, e, k, pass, eta2, tiny);
     return hipGetLastError();
 }
 
@@ -425,15 +427,11 @@ __global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const 
     if (threadIdx.x == 0) e.part[(size_t)j * e.ni + item] = s2;
 }
 
-__global__ void eig_ritz_norm_kernel(EigArgs e, int nev, const int32_t* nf, double* n2) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= e.P * nev) return;
-    const int p = t / nev, j = t % nev;
+__global__ void __launch_bounds__(64) eig_ritz_norm_kernel(EigArgs e, int nev, const int32_t* nf, double* n2) {
+    const int p = blockIdx.x, j = blockIdx.y;
     if (j >= nf[p]) return;
-    const int i0 = e.pitem0[p], n = e.pnitem[p];
-    double s = 0.0;
-    for (int it = 0; it < n; ++it) s += e.part[(size_t)j * e.ni + i0 + it];
-    n2[t] = s;
+    const double s = wave_item_sum(e.part, (size_t)j * e.ni + e.pitem0[p], e.pnitem[p]);
+    if (threadIdx.x == 0) n2[(size_t)p * nev + j] = s;
 }
 
 __global__ void __launch_bounds__(kEigThreads) eig_ritz_scale_kernel(EigArgs e, int nev, const int32_t* nf,
@@ -452,7 +450,7 @@ hipError_t launch_eig_ritz(const EigArgs& e, const double* V, const double* S, i
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s) {
     hipLaunchKernelGGL(eig_ritz_kernel, dim3(e.ni, nev), dim3(kEigThreads), (size_t)lds * sizeof(double), s, e, V, S,
                        lds, nev, nf, out);
-    hipLaunchKernelGGL(eig_ritz_norm_kernel, dim3((e.P * nev + 63) / 64), dim3(64), 0, s, e, nev, nf, n2);
+    hipLaunchKernelGGL(eig_ritz_norm_kernel, dim3(e.P, nev), dim3(64), 0, s, e, nev, nf, n2);
     hipLaunchKernelGGL(eig_ritz_scale_kernel, dim3(e.ni, nev), dim3(kEigThreads), 0, s, e, nev, nf, out, n2);
     return hipGetLastError();
 }

@@ -184,18 +184,6 @@ hipError_t launch_lz_update(int j, const double* w, const double* u1, const doub
 hipError_t launch_lz_finish_b(int j, const double* part, int npart, LzScal sc, hipStream_t s);
 hipError_t launch_lz_finish_a(int j, const double* part, int npart, LzScal sc, double thr, hipStream_t s);
 
-// full-reorthogonalisation Lanczos for eigenpairs: V holds unnormalised u_0..u_k (column stride ld)
-// with norms nrm[0..k]; pass 1: h[i] = <u_i, w>, and |w|^2; pass 2: w' = w - sum_i c_i u_i -> V[:,k+1], |w'|^2
-hipError_t launch_multidot(const double* V, int64_t ld, int k1, const double* w, int64_t n, double* part,
-                           int nblk, hipStream_t s);
-hipError_t launch_multidot_final(const double* part, int k1, int nblk, double* h, hipStream_t s);
-hipError_t launch_project_out(const double* V, int64_t ld, int k1, const double* c, const double* w, double* out,
-                              int64_t n, double* part, int nblk, hipStream_t s);
-int multidot_blocks(int64_t n);
-// Ritz vectors: Y[:, i] = sum_k S[k, i] V[:, k] for i < nev (S column-major m x nev, already divided by nrm)
-hipError_t launch_ritz(const double* V, int64_t ld, int m, const double* S, int nev, double* Y, int64_t ldy,
-                       int64_t n, hipStream_t s);
-
 // continued fraction over all frequencies, accumulating w*g into G (interleaved complex, stride)
 struct CfArgs {
     const double* a;

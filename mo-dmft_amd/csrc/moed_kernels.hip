@@ -237,144 +237,6 @@ hipError_t launch_lz_finish_a(int j, const double* part, int npart, LzScal sc, d
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------- reorthogonalisation
-
-constexpr int kMdRows = 16;  // rows per thread in the tall-skinny kernels
-constexpr int kMdThreads = 256;
-constexpr int kMdBlockRows = kMdRows * kMdThreads;
-
-int multidot_blocks(int64_t n) { return (int)((n + kMdBlockRows - 1) / kMdBlockRows); }
-
-// part[i*nblk + blk] = sum over the block's rows of u_i * w (i < k1), part[k1*nblk + blk] = sum w^2
-__global__ void __launch_bounds__(kMdThreads) multidot_kernel(const double* V, int64_t ld, int k1, const double* w,
-                                                             int64_t n, double* part, int nblk) {
-    extern __shared__ double red[];  // (k1+1) * 4
-    const int64_t base = (int64_t)blockIdx.x * kMdBlockRows + threadIdx.x;
-    double wv[kMdRows];
-    double ww = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMdRows; ++q) {
-        const int64_t r = base + (int64_t)q * kMdThreads;
-        wv[q] = r < n ? w[r] : 0.0;
-        ww += wv[q] * wv[q];
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = 0; i <= k1; ++i) {
-        double s;
-        if (i < k1) {
-            const double* u = V + (size_t)i * ld;
-            s = 0.0;
-#pragma unroll
-            for (int q = 0; q < kMdRows; ++q) {
-                const int64_t r = base + (int64_t)q * kMdThreads;
-                if (r < n) s += u[r] * wv[q];
-            }
-        } else {
-            s = ww;
-        }
-        s = wave_sum(s);
-        if (lane == 0) red[i * 4 + wid] = s;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i <= k1; i += kMdThreads)
-        part[(size_t)i * nblk + blockIdx.x] = ((red[i * 4] + red[i * 4 + 1]) + red[i * 4 + 2]) + red[i * 4 + 3];
-}
-
-hipError_t launch_multidot(const double* V, int64_t ld, int k1, const double* w, int64_t n, double* part, int nblk,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(multidot_kernel, dim3(nblk), dim3(kMdThreads), (size_t)(k1 + 1) * 4 * sizeof(double), s, V, ld,
-                       k1, w, n, part, nblk);
-    return hipGetLastError();
-}
-
-// h[i] = sum_blk part[i*nblk + blk] for i <= k1, one block per i
-__global__ void __launch_bounds__(256) multidot_final_kernel(const double* part, int nblk, double* h) {
-    __shared__ double red[4];
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += 256) s += part[(size_t)blockIdx.x * nblk + b];
-    s = block_sum<256>(s, red);
-    if (threadIdx.x == 0) h[blockIdx.x] = s;
-}
-
-hipError_t launch_multidot_final(const double* part, int k1, int nblk, double* h, hipStream_t s) {
-    hipLaunchKernelGGL(multidot_final_kernel, dim3(k1 + 1), dim3(256), 0, s, part, nblk, h);
-    return hipGetLastError();
-}
-
-// out = w - sum_i c_i u_i, part[blk] = block sum of out^2
-__global__ void __launch_bounds__(kMdThreads) project_out_kernel(const double* V, int64_t ld, int k1, const double* c,
-                                                                const double* w, double* out, int64_t n,
-                                                                double* part) {
-    extern __shared__ double sc[];
-    __shared__ double red[4];
-    for (int i = threadIdx.x; i < k1; i += kMdThreads) sc[i] = c[i];
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kMdBlockRows + threadIdx.x;
-    double acc[kMdRows];
-#pragma unroll
-    for (int q = 0; q < kMdRows; ++q) {
-        const int64_t r = base + (int64_t)q * kMdThreads;
-        acc[q] = r < n ? w[r] : 0.0;
-    }
-    for (int i = 0; i < k1; ++i) {
-        const double* u = V + (size_t)i * ld;
-        const double ci = sc[i];
-#pragma unroll
-        for (int q = 0; q < kMdRows; ++q) {
-            const int64_t r = base + (int64_t)q * kMdThreads;
-            if (r < n) acc[q] -= ci * u[r];
-        }
-    }
-    double s = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMdRows; ++q) {
-        const int64_t r = base + (int64_t)q * kMdThreads;
-        if (r < n) {
-            out[r] = acc[q];
-            s += acc[q] * acc[q];
-        }
-    }
-    s = block_sum<kMdThreads>(s, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-hipError_t launch_project_out(const double* V, int64_t ld, int k1, const double* c, const double* w, double* out,
-                              int64_t n, double* part, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(project_out_kernel, dim3(nblk), dim3(kMdThreads), (size_t)std::max(k1, 1) * sizeof(double), s,
-                       V, ld, k1, c, w, out, n, part);
-    return hipGetLastError();
-}
-
-// Y[:, i] = sum_k S[k + m*i] V[:, k]
-__global__ void __launch_bounds__(256) ritz_kernel(const double* V, int64_t ld, int m, const double* S, int nev,
-                                                   double* Y, int64_t ldy, int64_t n) {
-    extern __shared__ double sS[];
-    for (int i = threadIdx.x; i < m * nev; i += 256) sS[i] = S[i];
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r >= n) return;
-    double acc[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
-    for (int k = 0; k < m; ++k) {
-        const double v = V[(size_t)k * ld + r];
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (i < nev) acc[i] += sS[k + m * i] * v;
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-        if (i < nev) Y[(size_t)i * ldy + r] = acc[i];
-}
-
-hipError_t launch_ritz(const double* V, int64_t ld, int m, const double* S, int nev, double* Y, int64_t ldy, int64_t n,
-                       hipStream_t s) {
-    if (nev > 16) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ritz_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), (size_t)m * nev * sizeof(double), s,
-                       V, ld, m, S, nev, Y, ldy, n);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------- continued fraction
 
 // green_diag's continued fraction (ed_green.F90:160-168, 176-184) at one frequency per thread, with
