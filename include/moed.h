@@ -145,6 +145,17 @@ int moed_chi2_grad(moed_handle* h, int32_t nfit, int32_t nx, const double* x, in
 int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t nw, const double* omega,
                   const double* Dev, double ftol, double* fret, int32_t* iter, int32_t* status);
 
+/* Lattice step of the DMFT loop (next to the solver: main.f90:76-79).
+ * moed_band_tb: the diagonal of H(k) of ed_set_band_structure / find_hk (ed_hamiltonian.F90:29-85),
+ *   hk[q*norb + o] = -0.5 (cos kx + cos ky) on the nk x nk grid, nk = int(sqrt(nq)); host only.
+ * moed_delta_new: ed_delta_new + n_from_gksum (ed_green.F90:187-253) on the device:
+ *   Gksum(o,w) = (1/Nq) sum_q 1/((1/Gr + Dev + rmu) - hk(q,o)),  Dev <- 1/Gr - 1/Gksum + Dev (in place),
+ *   nocc[o] = 2 sum_w Re Gksum(o,w) / beta + 1/2.  Gr, Dev, Gksum: complex (norb, nw) column-major;
+ *   Gksum and nocc may be NULL. */
+int moed_band_tb(int32_t norb, int32_t nq, double* hk);
+int moed_delta_new(moed_handle* h, int32_t nw, int32_t nq, double rmu, const double* hk, const double* Gr,
+                   double* Dev, double* Gksum, double* nocc);
+
 /* Matsubara grid of ed_green_init (ed_green.F90:45-51) at one rank. */
 int moed_omega_grid(int32_t nw, double beta, double* omega);
 

@@ -1320,6 +1320,55 @@ int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t n
     return fit_minimize_host(h, nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
 }
 
+int moed_band_tb(int32_t norb, int32_t nq, double* hk) {
+    if (norb < 1 || nq < 1 || !hk) return MOED_EARG;
+    const double pi = 4.0 * std::atan(1.0);
+    const int nk = (int)std::sqrt((double)nq);
+    if (nk * nk != nq) return MOED_EARG;  // the reference stops unless nk*nk = Nq (ed_hamiltonian.F90:51)
+    const double dq = 2.0 * pi / (double)(float)nk;
+    int ik = 0;
+    for (int i = 1; i <= nk; ++i)
+        for (int j = 1; j <= nk; ++j, ++ik) {
+            const double kx = dq * (double)(float)(i - 1), ky = dq * (double)(float)(j - 1);
+            const double v = -0.5 * (std::cos(kx) + std::cos(ky));
+            for (int o = 0; o < norb; ++o) hk[(size_t)ik * norb + o] = v;
+        }
+    return MOED_OK;
+}
+
+int moed_delta_new(moed_handle* h, int32_t nw, int32_t nq, double rmu, const double* hk, const double* Gr,
+                   double* Dev, double* Gksum, double* nocc) {
+    if (!h) return MOED_EARG;
+    if (nw < 1 || nq < 1 || !hk || !Gr || !Dev) return fail(h, MOED_EARG, "delta_new: bad arguments");
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    const int norb = h->norb;
+    const size_t ncx = (size_t)2 * norb * nw;
+    DevBuf b;
+    HIPCHK(b.ensure(((size_t)nq * norb + 3 * ncx) * sizeof(double)));
+    double* d_hk = b.as<double>();
+    double* d_gr = d_hk + (size_t)nq * norb;
+    double* d_dev = d_gr + ncx;
+    double* d_gks = d_dev + ncx;
+    HIPCHK(hipMemcpyAsync(d_hk, hk, (size_t)nq * norb * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_gr, Gr, ncx * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_dev, Dev, ncx * sizeof(double), hipMemcpyHostToDevice, st));
+    DeltaArgs a{norb, nw, nq, rmu, d_hk, d_gr, d_dev, d_gks};
+    HIPCHK(launch_delta_new(a, st));
+    std::vector<double> gks(ncx);
+    HIPCHK(hipMemcpyAsync(Dev, d_dev, ncx * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(gks.data(), d_gks, ncx * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (Gksum) std::copy(gks.begin(), gks.end(), Gksum);
+    if (nocc)
+        for (int o = 0; o < norb; ++o) {
+            double sum = 0.0;
+            for (int w = 0; w < nw; ++w) sum = sum + gks[2 * ((size_t)w * norb + o)];
+            nocc[o] = 2.0 * sum / h->beta + 0.50;
+        }
+    return MOED_OK;
+}
+
 int moed_omega_grid(int32_t nw, double beta, double* omega) {
     if (nw < 0 || !omega || !(beta > 0)) return MOED_EARG;
     const double pi = 4.0 * std::atan(1.0);

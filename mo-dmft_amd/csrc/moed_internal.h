@@ -202,6 +202,17 @@ hipError_t launch_cf(const CfArgs& a, hipStream_t s);
 hipError_t launch_cf_adjoint(const CfArgs& a, const double* gbar, double* scratch, double* ga, double* gb,
                              hipStream_t s);
 
+// lattice step ed_delta_new (ed_green.F90:187-215); complex arrays interleaved, column-major (norb, nw)
+struct DeltaArgs {
+    int32_t norb, nw, nq;
+    double rmu;
+    const double* hk;  // [nq][norb] diagonal of H(k)
+    const double* gr;
+    double* dev;       // in: D_ev, out: new D_ev
+    double* gksum;     // optional
+};
+hipError_t launch_delta_new(const DeltaArgs& a, hipStream_t s);
+
 // batched FUNC + dfunc (func.f90:1-142) in the reference's summation order
 struct FitArgs {
     const double* x;  // [nfit][nx]

@@ -428,4 +428,43 @@ hipError_t launch_fit(const FitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- lattice step
+
+// ed_delta_new (ed_green.F90:187-215): one thread per (orbital blockIdx.y, frequency).  The k-sum
+// runs over q in order; 1/Gr + D_ev + rMu is the same for every q and is formed once.  Gksum /
+// float(Nq) is the complex division by (Nq, 0), as the reference build evaluates it.
+__global__ void __launch_bounds__(256) delta_new_kernel(DeltaArgs a) {
+#pragma clang fp contract(off)
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    const int o = blockIdx.y;
+    if (iw >= a.nw) return;
+    const size_t e = (size_t)iw * a.norb + o;
+    const cxd G = {a.gr[2 * e], a.gr[2 * e + 1]};
+    const cxd D = {a.dev[2 * e], a.dev[2 * e + 1]};
+    const cxd g1 = cx_rdiv(1.0, G);
+    const cxd base = {(g1.re + D.re) + a.rmu, (g1.im + D.im) + 0.0};
+    const cptr<double> hk = as_const(a.hk);
+    cxd s = {0.0, 0.0};
+    for (int q = 0; q < a.nq; ++q) {
+        const cxd t = {base.re - hk[(size_t)q * a.norb + o], base.im - 0.0};
+        const cxd r = cx_rdiv(1.0, t);
+        s = {s.re + r.re, s.im + r.im};
+    }
+    s = cx_div(s, {(double)(float)a.nq, 0.0});
+    if (a.gksum) {
+        a.gksum[2 * e] = s.re;
+        a.gksum[2 * e + 1] = s.im;
+    }
+    const cxd gi = cx_rdiv(1.0, s);
+    const cxd dn = {(g1.re - gi.re) + D.re, (g1.im - gi.im) + D.im};
+    a.dev[2 * e] = dn.re;
+    a.dev[2 * e + 1] = dn.im;
+}
+
+hipError_t launch_delta_new(const DeltaArgs& a, hipStream_t s) {
+    if (a.nw <= 0 || a.norb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(delta_new_kernel, dim3((a.nw + 255) / 256, a.norb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace moed

@@ -57,7 +57,7 @@ EXPORTS = [
     "moed_hv", "moed_hv_device", "moed_apply_c", "moed_lanczos", "moed_solve", "moed_sector_eigenvalues",
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
-    "moed_fill_start_device", "moed_set_hv_shape", "moed_stream",
+    "moed_fill_start_device", "moed_set_hv_shape", "moed_stream", "moed_band_tb", "moed_delta_new",
 ]
 
 _lib = None
@@ -98,6 +98,8 @@ def lib():
         L.moed_omega_grid.argtypes = [ct.c_int32, ct.c_double, _dp]
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
+        L.moed_band_tb.argtypes = [ct.c_int32, ct.c_int32, _dp]
+        L.moed_delta_new.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_double, _dp, _dp, _dp, _dp, _dp]
         L.moed_stream.argtypes = [ct.c_void_p]
         L.moed_stream.restype = ct.c_void_p
         _lib = L
@@ -126,6 +128,15 @@ def omega_grid(nw, beta):
     if rc:
         raise MoedError(rc, "omega grid")
     return om
+
+
+def band_tb(norb, nq):
+    """diagonal of H(k) on the nk x nk grid (ed_set_band_structure), shape (nq, norb)"""
+    hk = np.empty(nq * norb)
+    rc = lib().moed_band_tb(norb, nq, _d(hk))
+    if rc:
+        raise MoedError(rc, "band_tb: nq must be a square")
+    return hk.reshape(nq, norb)
 
 
 class EDSolver:
@@ -318,6 +329,21 @@ class EDSolver:
         self._check(lib().moed_minimize(self.h, nfit, self.nx, _d(x), len(omega), _d(omega), _d(D), ftol, _d(fret),
                                         it.ctypes.data_as(_ip), st.ctypes.data_as(_ip)))
         return x, fret, it, st
+
+    # ------------------------------------------------------------ lattice step
+    def delta_new(self, Gr, Dev, hk, rmu):
+        """ed_delta_new + n_from_gksum on the device: returns (Gksum, new Dev, nocc)"""
+        Gr = np.asarray(Gr, np.complex128)
+        norb, nw = Gr.shape
+        hk = _f64(np.asarray(hk).reshape(-1))
+        nq = len(hk) // norb
+        d = _cplx_flat(Dev)
+        gks = np.empty(2 * norb * nw)
+        nocc = np.empty(norb)
+        self._check(lib().moed_delta_new(self.h, nw, nq, float(rmu), _d(hk), _d(_cplx_flat(Gr)), _d(d), _d(gks),
+                                         _d(nocc)))
+        tocx = lambda v: (v[0::2] + 1j * v[1::2]).reshape(norb, nw, order="F")
+        return tocx(gks), tocx(d), nocc
 
     def set_hv_shape(self, threads=0, strip=0):
         self._check(lib().moed_set_hv_shape(self.h, threads, strip))

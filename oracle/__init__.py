@@ -53,6 +53,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"oracle library not built: {LIB_PATH} (run __graft_entry__.build())")
         L = ct.CDLL(LIB_PATH)
+        L.or_band_tb.argtypes = [ct.c_int, ct.c_int, _dp]
+        L.or_delta_new.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, _dp, _dp, _dp, _dp]
+        L.or_n_from_gksum.argtypes = [ct.c_int, ct.c_int, ct.c_double, _dp, _dp]
         L.or_icom.restype = ct.c_int32
         L.or_icom.argtypes = [ct.c_int32, ct.c_int32]
         L.or_basis_make.argtypes = [ct.c_int32, ct.c_int32, ct.c_int32, ct.POINTER(_Basis)]
@@ -232,6 +235,31 @@ def calc_dev(model, ef, omega, compat=True):
     lib().or_calc_dev(model.c, _d(_f64(ef)), nw, _d(_f64(omega)), int(compat), _d(D))
     D = D[0::2] + 1j * D[1::2]
     return D.reshape(model.norb, nw, order="F")
+
+
+def band_tb(norb, nq):
+    """Hk(o,o,q) of ed_set_band_structure (ed_hamiltonian.F90:29-85), shape (nq, norb)"""
+    hk = np.empty(nq * norb)
+    lib().or_band_tb(norb, nq, _d(hk))
+    return hk.reshape(nq, norb)
+
+
+def delta_new(Gr, Dev, hk, rmu):
+    """ed_delta_new (ed_green.F90:187-215): returns (Gksum, D_ev_new), complex (norb, nw)"""
+    norb, nw = Gr.shape
+    nq = hk.shape[0]
+    d = _dev_flat(Dev)
+    gks = np.empty(2 * norb * nw)
+    lib().or_delta_new(norb, nw, nq, float(rmu), _d(_f64(hk.reshape(-1))), _d(_dev_flat(Gr)), _d(d), _d(gks))
+    tocx = lambda v: (v[0::2] + 1j * v[1::2]).reshape(norb, nw, order="F")
+    return tocx(gks), tocx(d)
+
+
+def n_from_gksum(Gksum, beta):
+    norb, nw = Gksum.shape
+    out = np.empty(norb)
+    lib().or_n_from_gksum(norb, nw, float(beta), _d(_dev_flat(Gksum)), _d(out))
+    return out
 
 
 def _dev_flat(Dev):

@@ -1020,3 +1020,54 @@ done:
     free(xt);
     return rc;
 }
+
+/* ---------------------------------------------------------------- lattice step */
+
+/* ed_set_band_structure + find_hk (ed_hamiltonian.F90:29-85) */
+void or_band_tb(int norb, int nq, double* hk) {
+    const double pi = 4.0 * atan(1.0);                  /* ed_config.f90:7 */
+    const int nk = (int)sqrt((double)nq);               /* int(dsqrt(dfloat(Nq))) */
+    const double dq = 2.0 * pi / (double)(float)nk;     /* float(nk): REAL*4, exact */
+    int ik = 0;
+    for (int i = 1; i <= nk; i++)
+        for (int j = 1; j <= nk; j++) {
+            const double kx = dq * (double)(float)(i - 1);
+            const double ky = dq * (double)(float)(j - 1);
+            const double h = -0.5 * (cos(kx) + cos(ky));
+            for (int o = 0; o < norb; o++) hk[(size_t)ik * norb + o] = h;
+            ik++;
+        }
+}
+
+/* ed_delta_new (ed_green.F90:187-215), node-local frequency slice = all nw */
+void or_delta_new(int norb, int nw, int nq, double rmu, const double* hk, const double* gr, double* dev,
+                  double* gksum) {
+    for (int iw = 0; iw < nw; iw++)
+        for (int o = 0; o < norb; o++) {
+            const size_t e = (size_t)iw * norb + o;
+            const cx G = cx_mk(gr[2 * e], gr[2 * e + 1]);
+            const cx D = cx_mk(dev[2 * e], dev[2 * e + 1]);
+            cx s = cx_mk(0.0, 0.0);
+            for (int q = 0; q < nq; q++) {
+                const cx tmp = cx_sub(cx_add(cx_add(cx_rdiv(1.0, G), D), cx_mk(rmu, 0.0)),
+                                      cx_mk(hk[(size_t)q * norb + o], 0.0));
+                s = cx_add(s, cx_rdiv(1.0, tmp));
+            }
+            const double fq = (double)(float)nq;
+            s = cx_div(s, cx_mk(fq, 0.0));
+            gksum[2 * e] = s.re;
+            gksum[2 * e + 1] = s.im;
+            const cx dn = cx_add(cx_sub(cx_rdiv(1.0, G), cx_rdiv(1.0, s)), D);
+            dev[2 * e] = dn.re;
+            dev[2 * e + 1] = dn.im;
+        }
+}
+
+/* n_from_gksum (ed_green.F90:222-253) */
+void or_n_from_gksum(int norb, int nw, double beta, const double* gksum, double* nocc) {
+    for (int o = 0; o < norb; o++) {
+        double s = 0.0;
+        for (int iw = 0; iw < nw; iw++) s = s + gksum[2 * ((size_t)iw * norb + o)];
+        nocc[o] = 2.0 * s / beta + 0.50;
+    }
+}

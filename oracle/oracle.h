@@ -103,6 +103,17 @@ void or_dfunc(const double* x, double* p, int nw, int ns, int nxsize, const doub
 int or_frprmn(double* p, int n, double ftol, int* iter, double* fret, int nw, int ns,
               const double* omega, const double* Dev, int nbath, int norb, int compat, int* nfev);
 
+/* lattice step.  ed_set_band_structure / find_hk (ed_hamiltonian.F90:29-85): hk[q*norb + o] =
+ * Hk(o,o,q) = -0.5 (cos kx + cos ky) on the nk x nk grid, nk = int(sqrt(Nq)).
+ * ed_delta_new (ed_green.F90:187-215): Gksum = (1/Nq) sum_q 1/((1/Gr + D_ev + rMu) - Hk(q)),
+ * D_ev <- 1/Gr - 1/Gksum + D_ev; gr, dev, gksum interleaved complex, column-major (norb, nw).
+ * Gksum/float(Nq) is the complex division by (Nq, 0), as the reference build evaluates it.
+ * n_from_gksum (ed_green.F90:222-253): nocc[o] = 2 sum_w Re Gksum(o, w) / beta + 0.5 */
+void or_band_tb(int norb, int nq, double* hk);
+void or_delta_new(int norb, int nw, int nq, double rmu, const double* hk, const double* gr, double* dev,
+                  double* gksum);
+void or_n_from_gksum(int norb, int nw, double beta, const double* gksum, double* nocc);
+
 #ifdef __cplusplus
 }
 #endif

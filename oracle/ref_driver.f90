@@ -18,6 +18,9 @@
 !   6 SORT        (func.f90:306)            i1=n, payload a(n) -> indx(n)
 !   7 H columns   (multiply_H on unit vectors) -> for each column: nnz, rows(nnz), vals(nnz)
 !   8 timing      i1 = calls; every MPI rank owns its row block -> seconds per call, ranks, y(1)
+!   9 ed_delta_new (ed_green.F90:187) after ed_set_band_structure (ed_hamiltonian.F90:29)
+!                 i1 = nw, i2 = Nq; payload rMu, Gr(2*norb*nw), D_ev(2*norb*nw)
+!                 -> Hk(i,i,q) real parts (norb*Nq), Gksum(2*norb*nw), D_ev(2*norb*nw)
 program ref_driver
     use parallel_params
     use ed_config
@@ -25,6 +28,7 @@ program ref_driver
     use ed_operators
     use ed_hamiltonian
     use ed_lanczos
+    use ed_green, only: nwloc, Gr, D_ev, Gksum, ed_delta_new
     implicit none
     character(len=512) :: fin, fout
     integer :: hdr(8), task, ne_up, ne_dn, i1, i2, i3, i, j, nnz, nstep_calc, nx, nwl
@@ -143,6 +147,20 @@ program ref_driver
             write(11) rows(1:nnz)
             write(11) y(1:nnz)
         enddo
+    case (9)
+        nwloc = i1
+        Nq = i2
+        read(10) uj(1)
+        rMu = uj(1)
+        call ed_hamiltonian_init
+        call ed_set_band_structure
+        allocate(Gr(norb, nwloc), D_ev(norb, nwloc), Gksum(norb, nwloc))
+        read(10) Gr
+        read(10) D_ev
+        call ed_delta_new
+        write(11) ((dble(Hk(i, i, j)), i = 1, norb), j = 1, Nq)
+        write(11) Gksum
+        write(11) D_ev
     end select
     if (node .eq. 0) close(11)
     close(10)

@@ -11,6 +11,7 @@ Fixtures written (all small):
   ref_ns12.npz    reference multiply_H on the 12-site sector: summary (sums, sampled entries)
   ref_fit.npz     reference FUNC, dfunc and the minimization() result on a 2-orbital fit
   ref_sort.npz    reference SORT index of an array with ties
+  ref_delta.npz   reference ed_set_band_structure + ed_delta_new (lattice step) on 3 orbitals
   kat.json        notebook 9x9 known-answer eigenvalues (mathematica/MO-DMFT-ED.nb:1555-1592)
   e0.json         ground-state energies: dense eigh (8 sites), ARPACK eigsh on the reference-identical
                   H v (12 and 14 sites)
@@ -126,6 +127,26 @@ def gen_ref(cfgs):
     print("  reference-build fixtures written")
 
 
+def gen_ref_delta():
+    """ed_delta_new at 3 orbitals, 64 frequencies, Nq = 400 (20 x 20 k grid), rMu = 7"""
+    norb, nw, nq, rmu = 3, 64, 400, 7.0
+    rng = np.random.default_rng(4711)
+    om = oracle.omega_grid(nw, 200.0)
+    eps = rng.uniform(-1.0, 1.0, (norb, 1))
+    Gr = 1.0 / (1j * om[None, :] - eps) + 0.05 * (rng.standard_normal((norb, nw)) + 1j * rng.standard_normal((norb, nw)))
+    Dev = 0.3 * (rng.standard_normal((norb, nw)) - 1j * np.abs(rng.standard_normal((norb, nw))))
+    m = oracle.Model(norb, 12, 4.0, 0.6, np.zeros(norb + 12), np.zeros((norb, 12)))
+    raw = oracle.run_ref(9, m, i1=nw, i2=nq, payload=[np.array([rmu]), oracle._dev_flat(Gr), oracle._dev_flat(Dev)])
+    a = np.frombuffer(raw, "<f8")
+    tocx = lambda v: (v[0::2] + 1j * v[1::2]).reshape(norb, nw, order="F")
+    hk = a[:norb * nq].reshape(nq, norb)
+    gks = tocx(a[norb * nq:norb * nq + 2 * norb * nw])
+    dev_new = tocx(a[norb * nq + 2 * norb * nw:])
+    np.savez_compressed(os.path.join(HERE, "ref_delta.npz"), norb=norb, nw=nw, nq=nq, rmu=rmu, Gr=Gr, Dev=Dev,
+                        hk=hk, gksum=gks, dev_new=dev_new)
+    print("  reference lattice-step fixture written")
+
+
 def gen_dense(cfgs):
     kat = dict(source="mathematica/MO-DMFT-ED.nb:1498-1505 (parameters), :1555-1592 (matrix)",
                norb=2, nbath=1, U=400.0, J=200.0, ek=[-100.0, -100.0, 0.5], vk=[20.0, 10.0], sector=[2, 1],
@@ -159,10 +180,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--eigsh", action="store_true")
     ap.add_argument("--eigsh-big", action="store_true", help="also the 14-site sector (slow)")
+    ap.add_argument("--delta-only", action="store_true", help="only the lattice-step fixture")
     args = ap.parse_args()
     cfgs = load_configs()
+    if args.delta_only:
+        gen_ref_delta()
+        return
     if oracle.ref_available():
         gen_ref(cfgs)
+        gen_ref_delta()
     else:
         print("reference build not available; skipping reference fixtures")
     e0_path = os.path.join(HERE, "e0.json")

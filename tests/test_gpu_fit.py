@@ -78,3 +78,60 @@ def test_exact_fit_recovers_parameters():
         x, fret, it, st = s.minimization(xt + np.array([0.05, -0.05, 0.04, -0.03, 0.02]), om, Dev)
     assert st[0] == 0 and fret[0] < 1e-12
     assert np.allclose(np.abs(x[0]), np.abs(xt), atol=1e-4)
+
+
+# ------------------------------------------------------------------ lattice step and the DMFT loop
+
+def test_delta_new_bit_exact_with_reference():
+    """device lattice step (ed_delta_new) against the reference build, H(k) from moed_band_tb"""
+    import moed
+    z = np.load(os.path.join(GOLD, "ref_delta.npz"))
+    norb = int(z["norb"])
+    hk = moed.band_tb(norb, int(z["nq"]))
+    assert np.array_equal(hk, z["hk"])
+    with EDSolver(norb, 12, 4.0, 0.6, np.zeros(norb + 12), np.zeros((norb, 12)), beta=200.0) as s:
+        gks, dn, nocc = s.delta_new(z["Gr"], z["Dev"], hk, float(z["rmu"]))
+    assert np.array_equal(gks, z["gksum"]) and np.array_equal(dn, z["dev_new"])
+    assert np.allclose(nocc, oracle.n_from_gksum(z["gksum"], 200.0), rtol=0, atol=1e-12)
+
+
+def test_dmft_loop_steps_match_the_oracle():
+    """two iterations of the self-consistency loop on a 6-site, 2-orbital model: every step's
+    output against the oracle applied to the same inputs (lattice step bit-exact, fit to the
+    reference's CG trajectory within 1e-8, calc_dev bit-exact), and the ground state of the
+    second iteration's bath against dense ED"""
+    import moed
+    from moed import dmft
+    from oracle import dense_ed
+    rng = np.random.default_rng(12)
+    norb, nbath = 2, 4
+    ns = norb + nbath
+    ek = np.concatenate([np.full(norb, -2.0), rng.uniform(-2, 2, nbath)])
+    vk = np.zeros((norb, nbath))
+    for b in range(nbath):
+        vk[b % norb, b] = rng.uniform(0.3, 0.9)
+    U, J, beta, mu = 4.0, 0.5, 10.0, 2.0
+    om = omega_grid(128, beta)
+    hk = moed.band_tb(norb, 100)
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=beta, nstep=40, nev=2, sectors=[(3, 3)]) as s:
+        ef = ek[:norb].copy()
+        Dev = s.calc_dev(ek, vk, ef, om)
+        assert np.array_equal(Dev, oracle.calc_dev(oracle.Model(norb, nbath, U, J, ek, vk), ef, om))
+        for it in range(2):
+            kept = s.ed_solve()
+            sec = dense_ed.Sector(ns, 3, 3)
+            w = np.linalg.eigvalsh(dense_ed.hamiltonian(sec, norb, nbath, U, J, ek, vk).toarray())
+            assert abs(kept[0][0] - w[0]) <= 1e-10 * abs(w[0])
+            G = s.ed_calc_green_ftn(om)
+            _, Dn, _ = s.delta_new(G, Dev, hk, mu)
+            _, Dn_ref = oracle.delta_new(G, Dev, hk, mu)
+            assert np.array_equal(Dn, Dn_ref)
+            x0 = dmft.ev_to_x(ek, vk, ef, norb, nbath)
+            assert np.array_equal(x0, oracle.ev_to_x(ek, vk.reshape(-1, order="F"), ef, norb, nbath))
+            X, fret, _, st = s.minimization(x0, om, Dn)
+            rc, xr, fr, _, _ = oracle.frprmn(x0, om, Dn, norb, nbath)
+            assert st[0] == 0 and rc == 0
+            assert np.max(np.abs(X[0] - xr)) <= 1e-8 * max(1.0, np.abs(xr).max())
+            ef, ek, vk = dmft.x_to_ev(X[0], norb, nbath)
+            s.set_bath(ek, vk)
+            Dev = s.calc_dev(ek, vk, ef, om)

@@ -166,6 +166,32 @@ def test_func_dfunc_frprmn_match_reference_build():
     assert rc == 0 and fret == float(z["fret"]) and np.array_equal(x, z["xmin"])
 
 
+def test_delta_new_matches_reference_build():
+    """lattice step (band structure, k-sum, new hybridisation) against the reference build"""
+    z = np.load(os.path.join(GOLD, "ref_delta.npz"))
+    hk = oracle.band_tb(int(z["norb"]), int(z["nq"]))
+    assert np.array_equal(hk, z["hk"])
+    gks, dn = oracle.delta_new(z["Gr"], z["Dev"], hk, float(z["rmu"]))
+    assert np.array_equal(gks, z["gksum"]) and np.array_equal(dn, z["dev_new"])
+
+
+def test_delta_new_closed_forms():
+    """one k point with Hk = 0: Gksum = 1/(1/G + D + mu), so D_new = 1/G - 1/Gksum + D = -mu up to
+    rounding; flat band: the k-sum is the same k point repeated"""
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((2, 8)) - 1j * np.abs(rng.standard_normal((2, 8)))
+    D = rng.standard_normal((2, 8)) + 1j * rng.standard_normal((2, 8))
+    gks, dn = oracle.delta_new(G, D, np.zeros((1, 2)), 0.75)
+    assert np.allclose(gks, 1 / (1 / G + D + 0.75), rtol=1e-14)
+    assert np.allclose(dn, -0.75, atol=1e-12)
+    gks4, dn4 = oracle.delta_new(G, D, np.full((4, 2), 0.2), 0.75)
+    gks1, dn1 = oracle.delta_new(G, D, np.full((1, 2), 0.2), 0.75)
+    assert np.allclose(gks4, gks1, rtol=1e-14) and np.allclose(dn4, dn1, rtol=1e-13)
+    # band structure: cos grid, average zero on a full period
+    hk = oracle.band_tb(1, 100)
+    assert hk.shape == (100, 1) and abs(hk.mean()) < 1e-14 and hk.min() == -1.0
+
+
 def test_sort_matches_reference_build():
     z = np.load(os.path.join(GOLD, "ref_sort.npz"))
     assert np.array_equal(oracle.sort_index(z["a"]), z["indx"])
