@@ -43,6 +43,10 @@ extern "C" {
 #define MOED_F_NO_TUNING 4u     /* ignore the launch-shape tuning table */
 #define MOED_F_BATCHED 8u       /* moed_solve: all sectors in one multi-problem Lanczos run (one launch
                                    per operation for every sector) instead of sector by sector */
+#define MOED_F_FIT_TREE 16u     /* bath fit: sum chi^2 and its gradient over frequencies in a fixed tree
+                                   order instead of the reference's sequential order: deterministic and
+                                   much faster, rounded differently (the CG path then differs in the
+                                   last bits from the reference's) */
 
 typedef struct moed_handle moed_handle;
 
@@ -141,9 +145,17 @@ int moed_calc_dev(moed_handle* h, const double* ek, const double* vk, const doub
 int moed_chi2_grad(moed_handle* h, int32_t nfit, int32_t nx, const double* x, int32_t nw, const double* omega,
                    const double* Dev, double* f, double* g);
 /* minimization (func.f90:144-163 -> frprmn.f90) for nfit independent starting points at once:
- * x[nfit][nx] in/out, fret[nfit], iter[nfit], status[nfit] (0 ok, MOED_ECONV on itmax). */
+ * x[nfit][nx] in/out, fret[nfit], iter[nfit], status[nfit] (0 ok, MOED_ECONV on itmax, where the
+ * reference stops the program: frprmn.f90:83).  The whole minimiser runs on the device, one
+ * workgroup per fit; omega must be positive and omega, Dev, x finite. */
 int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t nw, const double* omega,
                   const double* Dev, double ftol, double* fret, int32_t* iter, int32_t* status);
+/* CG iterations each fit runs per kernel launch inside moed_minimize (0: default).  It bounds the
+ * duration of one launch; results do not depend on it.  No counterpart in the reference. */
+int moed_set_fit_launch(moed_handle* h, int32_t iterations);
+/* FUNC evaluations (f1dim calls plus the first FUNC) of each of the first nfit fits of the last
+ * moed_minimize.  No counterpart in the reference (diagnostics). */
+int moed_fit_nfev(moed_handle* h, int32_t nfit, int32_t* nfev);
 
 /* Lattice step of the DMFT loop (next to the solver: main.f90:76-79).
  * moed_band_tb: the diagonal of H(k) of ed_set_band_structure / find_hk (ed_hamiltonian.F90:29-85),

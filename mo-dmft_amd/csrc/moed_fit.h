@@ -1,5 +1,5 @@
 // Bath fit on the device: batched chi^2 and gradient (func.f90:1-142) and the conjugate-gradient
-// minimiser (frprmn.f90) run for many independent starting points in lockstep.
+// minimiser (frprmn.f90), one workgroup per independent starting point, all of them in one launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,6 +12,9 @@ namespace moed {
 struct FitCtx {
     hipStream_t stream;
     int norb, nbath, compat;
+    int tree;      // sums in a fixed tree order instead of the reference's sequential order
+    int budget;    // CG iterations per fit and launch (0: default)
+    int* nfev;     // optional [nfit]: function evaluations of each fit
     std::string* err;
 };
 

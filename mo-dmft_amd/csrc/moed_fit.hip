@@ -1,20 +1,590 @@
-// Bath fit driver: batched device evaluation of FUNC/dfunc and the reference's conjugate-gradient
-// minimiser restated as one C++20 coroutine per fit.  Every fit suspends at each function
-// evaluation; the driver gathers the pending points of all fits into one launch, so one kernel
-// pair evaluates up to nfit chi^2 (and gradients) at a time.
+// Bath fit on the device: chi^2 (FUNC) and its gradient (dfunc) of func.f90:1-142, and the whole
+// conjugate-gradient minimiser of frprmn.f90 (frprmn, linmin, mnbrak, brent, f1dim) for many
+// independent fits in one launch.
+//
+// One workgroup owns one fit for its whole life: the CG vectors (p, xi, g, h, pcom, xicom) sit in
+// LDS, every thread keeps the same copy of the line-search scalars, and each function evaluation
+// is a block-wide pass over the Matsubara frequencies.  Nothing returns to the host between
+// evaluations.  A launch runs at most `budget` CG iterations per fit and leaves the state at an
+// iteration boundary in HBM; the host relaunches until every fit has finished, so one launch stays
+// short and the result does not depend on the budget.
+//
+// Sums over frequencies come in two orders:
+//  * reference order (default): every sum runs sequentially in the reference's loop order, one
+//    lane per sum over terms staged in LDS, so chi^2, the gradient and the CG trajectory are
+//    bit-identical to the reference build;
+//  * tree order (MOED_F_FIT_TREE): per-thread partial sums over a fixed frequency stride, then a
+//    fixed wave/block tree.  Deterministic and independent of the batch, but rounded differently
+//    from the reference; far less latency per evaluation.
+// Couplings V(o,k) that are exactly zero are skipped: each of their terms is a signed zero, and a
+// running sum that starts at +0.0 is unchanged by adding signed zeros (inputs are finite).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
-#include <coroutine>
-#include <exception>
-#include <memory>
 #include <vector>
 
+#include "moed_cplx.h"
+#include "moed_device.h"
 #include "moed_fit.h"
 #include "moed_internal.h"
 
 namespace moed {
 namespace {
+
+constexpr int kFitThreads = 1024;           // 16 waves: four per SIMD hide the division latency
+constexpr int kFitNx = 64;                    // >= nxsize; the reference's nmax is 60 (frprmn.f90:12)
+constexpr int kFitMaxBath = kMaxSite - 1;
+constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
+constexpr int kFitChunkF = kFitThreads - 64;   // frequencies per staged FUNC chunk (waves 1..)
+constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
+constexpr int kItmax = 20000, kBrentItmax = 100;  // frprmn.f90:12, :126
+constexpr int kStateD = 3 * kFitNx + 2;       // per fit: xi, g, h, fp, fret
+constexpr int kStateI = 4;                    // per fit: phase, its, rc, nfev
+
+enum { kPhaseStart = 0, kPhaseRun = 1, kPhaseDone = 2 };
+
+struct FitProb {
+    int32_t nx, norb, nbath, nw, compat, tree;
+    const double* omega;
+    const double* dev;  // interleaved complex (norb, nw) column-major
+};
+
+struct FitCgArgs {
+    FitProb q;
+    int32_t nfit, budget;
+    double ftol;
+    double* x;       // [nfit][nx]: start point in, result out
+    double* st;      // [nfit][kStateD]
+    int32_t* ist;    // [nfit][kStateI]
+    int32_t* active; // fits still running after this launch
+};
+
+struct FitEvalArgs {
+    FitProb q;
+    const double* x;  // [nfit][nx]
+    double* f;        // [nfit] or null
+    double* g;        // [nfit][nx] or null
+};
+
+struct FitSmem {
+    double p[kFitNx], xi[kFitNx], g[kFitNx], h[kFitNx], pcom[kFitNx], xicom[kFitNx], xt[kFitNx];
+    double gacc[kFitNx];
+    double bcast[2];
+    double red[kFitThreads / 64];
+    double gred[kFitThreads / 64][kFitCols];      // per-wave column sums (tree gradient)
+    union {
+        double fstage[2][kFitChunkF * kMaxOrb];  // FUNC terms, two chunks (reference order)
+        double cols[kFitCols * kFitChunkG];      // gradient terms of one chunk (reference order)
+    } u;
+};
+
+// the fit's LDS, shared by every device function of this file (one workgroup = one fit)
+__shared__ FitSmem S;
+
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+__device__ __forceinline__ const double* uniform_ptr(const double* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const double*)(((uint64_t)hi << 32) | lo);
+}
+
+// the problem with every field wave-uniform (scalar registers) inside out-of-line functions
+struct FitView {
+    int nx, norb, nbath, nw, compat;
+    gptr<double> omega;
+    gptr<double> dev;
+};
+__device__ __forceinline__ FitView view(const FitProb& q) {
+    return FitView{uniform_i32(q.nx), uniform_i32(q.norb), uniform_i32(q.nbath), uniform_i32(q.nw),
+                   uniform_i32(q.compat), (gptr<double>)uniform_ptr(q.omega), (gptr<double>)uniform_ptr(q.dev)};
+}
+
+__device__ __forceinline__ double zim(const FitView& q, double w) { return q.compat ? round_f32(w) : w; }
+
+// Delta_o(i w) - Dev_o(w) at the point xt (func.f90:19-29): the k sum in the reference's order
+__device__ __forceinline__ cxd residual(const FitView& q, const double* xt, int o, int iw, double zi) {
+#pragma clang fp contract(off)
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < q.nbath; ++k) {
+        const double v = xt[q.nbath + o * q.nbath + k];
+        if (v == 0.0) continue;
+        const cxd t = cx_rdiv(v * v, {0.0 - xt[k], zi});
+        re = re + t.re;
+        im = im + t.im;
+    }
+    const size_t io = (size_t)o + (size_t)q.norb * iw;
+    return {(re + xt[q.nx - q.norb + o]) - q.dev[2 * io], im - q.dev[2 * io + 1]};
+}
+
+// s + buf[0] + buf[1] + ... + buf[n-1], one rounding per add in this order; the LDS loads run
+// eight elements ahead of the dependent adds
+__device__ __forceinline__ double chain_sum(double s, const double* buf, int n) {
+#pragma clang fp contract(off)
+    int j = 0;
+    if (n >= 16) {
+        double a[8], b[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[t] = buf[t];
+        for (j = 8; j + 8 <= n; j += 8) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) b[t] = buf[j + t];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) s = s + a[t];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) a[t] = b[t];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s = s + a[t];
+    }
+    for (; j < n; ++j) s = s + buf[j];
+    return s;
+}
+
+// FUNC (func.f90:30-42): sum over (w, o) of |r|^2 / w_n, divided by float(Norb*Norb*Nw).
+// Reference order: waves 1.. stage the terms of frequency chunk c while lane 0 of wave 0 adds chunk
+// c - 1 to the running sum (double-buffered).  Tree order: per-thread partials, then block_sum.
+__device__ __noinline__ double eval_func(const FitProb qa) {
+#pragma clang fp contract(off)
+    const FitView q = view(qa);
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    if (uniform_i32(qa.tree)) {
+        for (int iw = tid; iw < q.nw; iw += kFitThreads) {
+            const double w = q.omega[iw], zi = zim(q, w);
+            for (int o = 0; o < q.norb; ++o) {
+                const cxd r = residual(q, S.xt, o, iw, zi);
+                const double h = hypot_corrected(r.re, r.im);
+                s = s + h * h / w;
+            }
+        }
+        s = block_sum<kFitThreads>(s, S.red);
+    } else {
+        const int nch = (q.nw + kFitChunkF - 1) / kFitChunkF;
+        for (int c = 0; c <= nch; ++c) {
+            if (tid >= 64 && c < nch) {
+                const int iw = c * kFitChunkF + (tid - 64);
+                if (iw < q.nw) {
+                    double* st = S.u.fstage[c & 1] + (tid - 64) * q.norb;
+                    const double w = q.omega[iw], zi = zim(q, w);
+                    for (int o = 0; o < q.norb; ++o) {
+                        const cxd r = residual(q, S.xt, o, iw, zi);
+                        const double h = hypot_corrected(r.re, r.im);
+                        st[o] = h * h / w;
+                    }
+                }
+            }
+            if (tid == 0 && c > 0) {
+                const int n = min(kFitChunkF, q.nw - (c - 1) * kFitChunkF) * q.norb;
+                s = chain_sum(s, S.u.fstage[(c - 1) & 1], n);
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) S.bcast[0] = s / (double)(float)(q.norb * q.norb * q.nw);
+    __syncthreads();
+    return S.bcast[0];
+}
+
+// gradient terms of one (w, o) (func.f90:89-124): column k = d/de_k, nbath + k = d/dV(o,k),
+// 2 nbath = d/def_o.  The k loop is unrolled to kFitMaxBath so that a caller may keep its
+// per-column partial sums in registers.
+template <class Put>
+__device__ __forceinline__ void grad_terms(const FitView& q, const double* xt, int o, int iw, Put put) {
+#pragma clang fp contract(off)
+    const double w = q.omega[iw], zi = zim(q, w);
+    const cxd r = residual(q, xt, o, iw, zi);
+    const cxd cr = {r.re, -r.im};
+    const int nb = q.nbath;
+#pragma unroll
+    for (int k = 0; k < kFitMaxBath; ++k) {
+        if (k < nb) {
+            const double v = xt[nb + o * nb + k];
+            if (v != 0.0) {
+                const cxd den = {0.0 - xt[k], zi};
+                const cxd t = cx_div(cx_div(cx_mul(cx_mul(cr, {v, 0.0}), {v, 0.0}), den), den);
+                put(k, 2.0 * t.re / w);
+                const cxd u = cx_div(cx_mul(cr, {v, 0.0}), den);
+                const double tre = u.re + u.re;
+                put(kFitMaxBath + k, 2.0 * tre / w);
+            }
+        }
+    }
+    put(2 * kFitMaxBath, 2.0 * r.re / w);
+}
+
+// dfunc (func.f90:47-142) at xt into out[0:nx] (LDS).  Reference order: d/de_k runs over o, then w
+// (one chain through all orbital passes); d/dV(o,k) and d/def_o over w.
+__device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
+#pragma clang fp contract(off)
+    const FitView q = view(qa);
+    const int tid = threadIdx.x, nb = q.nbath;
+    const double scale1 = (double)(float)(q.norb * q.norb), scale2 = (double)(float)q.nw;
+    {
+        const int lane = tid & 63, wid = tid >> 6;
+        for (int o = 0; o < q.norb; ++o) {
+            double part[kFitCols];  // columns k, kFitMaxBath + k, 2 kFitMaxBath: registers
+#pragma unroll
+            for (int c = 0; c < kFitCols; ++c) part[c] = 0.0;
+            for (int iw = tid; iw < q.nw; iw += kFitThreads) {
+                // keeps the parameter loads (LDS) inside the loop: hoisting all of them would
+                // push the register partials into scratch
+                asm volatile("" ::: "memory");
+                grad_terms(q, S.xt, o, iw, [&](int c, double t) { part[c] = part[c] + t; });
+            }
+            // fixed tree: wave sums, then the waves in index order
+#pragma unroll
+            for (int c = 0; c < kFitCols; ++c) {
+                const int k = c < kFitMaxBath ? c : c - kFitMaxBath;
+                if (c != 2 * kFitMaxBath && k >= nb) continue;
+                const double v = wave_sum(part[c]);
+                if (lane == 0) S.gred[wid][c] = v;
+            }
+            __syncthreads();
+            if (tid < kFitCols) {
+                const int c = tid, k = c < kFitMaxBath ? c : c - kFitMaxBath;
+                if (c == 2 * kFitMaxBath || k < nb) {
+                    double v = 0.0;
+                    for (int w = 0; w < kFitThreads / 64; ++w) v = v + S.gred[w][c];
+                    const bool zero_v = c != 2 * kFitMaxBath && S.xt[nb + o * nb + k] == 0.0;
+                    if (c < kFitMaxBath) {  // d/de_k: accumulated over the orbital passes
+                        if (o == 0)
+                            S.gacc[k] = zero_v ? 0.0 : v;
+                        else if (!zero_v)
+                            S.gacc[k] = S.gacc[k] + v;
+                    } else if (c < 2 * kFitMaxBath) {
+                        S.gacc[nb + o * nb + k] = zero_v ? 0.0 : v;
+                    } else {
+                        S.gacc[q.nx - q.norb + o] = v;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        for (int c = tid; c < q.nx; c += kFitThreads) out[c] = S.gacc[c] / scale1 / scale2;
+    }
+    __syncthreads();
+}
+
+__device__ __noinline__ void eval_grad_ref(const FitProb qa, double* out) {
+#pragma clang fp contract(off)
+    const FitView q = view(qa);
+    const int tid = threadIdx.x, nb = q.nbath;
+    const double scale1 = (double)(float)(q.norb * q.norb), scale2 = (double)(float)q.nw;
+    {
+        double s = 0.0;  // lane c < nx of wave 0: running sum of component c
+        for (int o = 0; o < q.norb; ++o) {
+            // staged column of this lane's component in this pass (-1: none), and whether it has terms
+            int col = -1;
+            if (tid < nb)
+                col = tid;
+            else if (tid >= nb + o * nb && tid < nb + (o + 1) * nb)
+                col = kFitMaxBath + (tid - nb - o * nb);
+            else if (tid == q.nx - q.norb + o)
+                col = 2 * kFitMaxBath;
+            const int k = col < kFitMaxBath ? col : col - kFitMaxBath;
+            const bool on = col >= 0 && (col == 2 * kFitMaxBath || S.xt[nb + o * nb + k] != 0.0);
+            for (int base = 0; base < q.nw; base += kFitChunkG) {
+                const int iw = base + tid;
+                if (tid < kFitChunkG && iw < q.nw)
+                    grad_terms(q, S.xt, o, iw, [&](int c, double t) { S.u.cols[c * kFitChunkG + tid] = t; });
+                __syncthreads();
+                if (on) s = chain_sum(s, S.u.cols + col * kFitChunkG, min(kFitChunkG, q.nw - base));
+                __syncthreads();
+            }
+        }
+        if (tid < q.nx) out[tid] = s / scale1 / scale2;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void eval_grad(const FitProb& q, double* out) {
+    if (uniform_i32(q.tree))
+        eval_grad_tree(q, out);
+    else
+        eval_grad_ref(q, out);
+}
+
+// f1dim (frprmn.f90:212-234)
+__device__ __forceinline__ double f1dim(const FitProb& q, double xv, int& nfev) {
+#pragma clang fp contract(off)
+    for (int j = threadIdx.x; j < q.nx; j += kFitThreads) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
+    __syncthreads();
+    ++nfev;
+    return eval_func(q);
+}
+
+__device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// mnbrak (frprmn.f90:236-310).  The reference's bracketing loop has no count limit; on a flat or
+// unbounded direction its steps grow geometrically until f turns NaN and the loop ends.  The cap
+// below is far beyond that and only turns a pathological input into an error instead of an endless
+// kernel; false when it is reached.
+constexpr int kBracketCap = 100000;
+__device__ bool mnbrak(const FitProb q, double& ax, double& bx, double& cx, double& fa, double& fb,
+                       double& fc, int& nfev) {
+#pragma clang fp contract(off)
+    const double GOLD = (double)1.618034f, GLIMIT = 100.0, TINY = (double)1.e-20f;
+    double dum, fu, q_, r, u, ulim;
+    fa = f1dim(q, ax, nfev);
+    fb = f1dim(q, bx, nfev);
+    if (fb > fa) {
+        dum = ax;
+        ax = bx;
+        bx = dum;
+        dum = fb;
+        fb = fa;
+        fa = dum;
+    }
+    cx = bx + GOLD * (bx - ax);
+    fc = f1dim(q, cx, nfev);
+    for (int guard = 0; fb >= fc; ++guard) {
+        if (guard == kBracketCap) return false;
+        r = (bx - ax) * (fb - fc);
+        q_ = (bx - cx) * (fb - fa);
+        u = bx - ((bx - cx) * q_ - (bx - ax) * r) / (2. * fsign(fmax(fabs(q_ - r), TINY), q_ - r));
+        ulim = bx + GLIMIT * (cx - bx);
+        if ((bx - u) * (u - cx) > 0.) {
+            fu = f1dim(q, u, nfev);
+            if (fu < fc) {
+                ax = bx;
+                fa = fb;
+                bx = u;
+                fb = fu;
+                return true;
+            } else if (fu > fb) {
+                cx = u;
+                fc = fu;
+                return true;
+            }
+            u = cx + GOLD * (cx - bx);
+            fu = f1dim(q, u, nfev);
+        } else if ((cx - u) * (u - ulim) > 0.) {
+            fu = f1dim(q, u, nfev);
+            if (fu < fc) {
+                bx = cx;
+                cx = u;
+                u = cx + GOLD * (cx - bx);
+                fb = fc;
+                fc = fu;
+                fu = f1dim(q, u, nfev);
+            }
+        } else if ((u - ulim) * (ulim - cx) >= 0.) {
+            u = ulim;
+            fu = f1dim(q, u, nfev);
+        } else {
+            u = cx + GOLD * (cx - bx);
+            fu = f1dim(q, u, nfev);
+        }
+        ax = bx;
+        bx = cx;
+        cx = u;
+        fa = fb;
+        fb = fc;
+        fc = fu;
+    }
+    return true;
+}
+
+// brent (frprmn.f90:119-210); false when its iteration limit is reached
+__device__ bool brent(const FitProb q, double ax, double bx, double cx, double tol, double& xmin,
+                      double& fret, int& nfev) {
+#pragma clang fp contract(off)
+    const double CGOLD = (double).3819660f, ZEPS = (double)1.0e-10f;
+    double a = fmin(ax, cx), b = fmax(ax, cx);
+    double d = 0.0, e = 0., etemp, fu, fv, fw, fx, p, q_, r, tol1, tol2, u, v, w, x, xm;
+    v = bx;
+    w = v;
+    x = v;
+    fx = f1dim(q, x, nfev);
+    fv = fx;
+    fw = fx;
+    for (int iter = 1; iter <= kBrentItmax; ++iter) {
+        xm = 0.5 * (a + b);
+        tol1 = tol * fabs(x) + ZEPS;
+        tol2 = 2. * tol1;
+        if (fabs(x - xm) <= (tol2 - .5 * (b - a))) {
+            xmin = x;
+            fret = fx;
+            return true;
+        }
+        bool golden = true;
+        if (fabs(e) > tol1) {
+            r = (x - w) * (fx - fv);
+            q_ = (x - v) * (fx - fw);
+            p = (x - v) * q_ - (x - w) * r;
+            q_ = 2. * (q_ - r);
+            if (q_ > 0.) p = -p;
+            q_ = fabs(q_);
+            etemp = e;
+            e = d;
+            if (!(fabs(p) >= fabs(.5 * q_ * etemp) || p <= q_ * (a - x) || p >= q_ * (b - x))) {
+                d = p / q_;
+                u = x + d;
+                if (u - a < tol2 || b - u < tol2) d = fsign(tol1, xm - x);
+                golden = false;
+            }
+        }
+        if (golden) {
+            e = (x >= xm) ? a - x : b - x;
+            d = CGOLD * e;
+        }
+        u = (fabs(d) >= tol1) ? x + d : x + fsign(tol1, d);
+        fu = f1dim(q, u, nfev);
+        if (fu <= fx) {
+            if (u >= x)
+                a = x;
+            else
+                b = x;
+            v = w;
+            fv = fw;
+            w = x;
+            fw = fx;
+            x = u;
+            fx = fu;
+        } else {
+            if (u < x)
+                a = u;
+            else
+                b = u;
+            if (fu <= fw || w == x) {
+                v = w;
+                fv = fw;
+                w = u;
+                fw = fu;
+            } else if (fu <= fv || v == x || v == w) {
+                v = u;
+                fv = fu;
+            }
+        }
+    }
+    return false;
+}
+
+// frprmn (frprmn.f90:1-84) with linmin (:86-117) inlined, for fit blockIdx.x
+__global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
+#pragma clang fp contract(off)
+    const FitProb q = a.q;
+    const int f = blockIdx.x, tid = threadIdx.x, n = q.nx;
+    int32_t* ist = a.ist + (size_t)f * kStateI;
+    double* st = a.st + (size_t)f * kStateD;
+    double* px = a.x + (size_t)f * n;
+    const int phase = uniform_i32(ist[0]);
+    if (phase == kPhaseDone) return;
+    const double EPS = 1.e-10, LTOL = (double)1.e-8f;  // frprmn.f90:12, :91
+    int its0 = uniform_i32(ist[1]), nfev = uniform_i32(ist[3]);
+    double fp = st[3 * kFitNx], fret = st[3 * kFitNx + 1];
+    for (int j = tid; j < n; j += kFitThreads) {
+        S.p[j] = px[j];
+        S.xi[j] = st[j];
+        S.g[j] = st[kFitNx + j];
+        S.h[j] = st[2 * kFitNx + j];
+        S.xt[j] = px[j];
+    }
+    __syncthreads();
+    if (phase == kPhaseStart) {
+        fp = eval_func(q);
+        ++nfev;
+        eval_grad(q, S.xi);
+        for (int j = tid; j < n; j += kFitThreads) {
+            S.g[j] = -S.xi[j];
+            S.h[j] = S.g[j];
+            S.xi[j] = S.h[j];
+        }
+        __syncthreads();
+        its0 = 0;
+    }
+    int rc = 1, iter = its0;
+    for (int its = its0 + 1; its <= kItmax; ++its) {
+        iter = its;
+        // ---- linmin
+        for (int j = tid; j < n; j += kFitThreads) {
+            S.pcom[j] = S.p[j];
+            S.xicom[j] = S.xi[j];
+        }
+        __syncthreads();
+        double ax = 0., xx = 1., bx, fa, fx, fb, xmin = 0.0, fr = 0.0;
+        if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
+            rc = MOED_ECONV;  // 'brent exceed maximum iterations'
+            break;
+        }
+        fret = fr;
+        for (int j = tid; j < n; j += kFitThreads) {
+            S.xi[j] = xmin * S.xi[j];
+            S.p[j] = S.p[j] + S.xi[j];
+        }
+        __syncthreads();
+        // ---- end linmin
+        if (2.0 * fabs(fret - fp) <= a.ftol * (fabs(fret) + fabs(fp) + EPS)) {
+            rc = MOED_OK;
+            break;
+        }
+        fp = fret;
+        for (int j = tid; j < n; j += kFitThreads) S.xt[j] = S.p[j];
+        __syncthreads();
+        eval_grad(q, S.xi);
+        double gg = 0.0, dgg = 0.0;
+        for (int j = 0; j < n; ++j) {
+            gg = gg + S.g[j] * S.g[j];
+            dgg = dgg + (S.xi[j] + S.g[j]) * S.xi[j];
+        }
+        if (gg == 0.0) {
+            rc = MOED_OK;
+            break;
+        }
+        const double gam = dgg / gg;
+        __syncthreads();
+        for (int j = tid; j < n; j += kFitThreads) {
+            S.g[j] = -S.xi[j];
+            S.h[j] = S.g[j] + gam * S.h[j];
+            S.xi[j] = S.h[j];
+        }
+        __syncthreads();
+        if (its - its0 >= a.budget && its < kItmax) {  // suspend at the iteration boundary
+            for (int j = tid; j < n; j += kFitThreads) {
+                px[j] = S.p[j];
+                st[j] = S.xi[j];
+                st[kFitNx + j] = S.g[j];
+                st[2 * kFitNx + j] = S.h[j];
+            }
+            if (tid == 0) {
+                st[3 * kFitNx] = fp;
+                st[3 * kFitNx + 1] = fret;
+                ist[0] = kPhaseRun;
+                ist[1] = its;
+                ist[3] = nfev;
+                atomicAdd(a.active, 1);
+            }
+            return;
+        }
+    }
+    if (rc == 1) rc = MOED_ECONV;  // 'frprmn maximum iterations exceeded!'
+    for (int j = tid; j < n; j += kFitThreads) px[j] = S.p[j];
+    if (tid == 0) {
+        st[3 * kFitNx + 1] = fret;
+        ist[0] = kPhaseDone;
+        ist[1] = iter;
+        ist[2] = rc;
+        ist[3] = nfev;
+    }
+}
+
+// FUNC and/or dfunc at x[blockIdx.x]
+__global__ void __launch_bounds__(kFitThreads) fit_eval_kernel(FitEvalArgs a) {
+    const int f = blockIdx.x, n = a.q.nx;
+    for (int j = threadIdx.x; j < n; j += kFitThreads) S.xt[j] = a.x[(size_t)f * n + j];
+    __syncthreads();
+    if (a.f) {
+        const double v = eval_func(a.q);
+        if (threadIdx.x == 0) a.f[f] = v;
+    }
+    if (a.g) {
+        eval_grad(a.q, S.g);
+        for (int j = threadIdx.x; j < n; j += kFitThreads) a.g[(size_t)f * n + j] = S.g[j];
+    }
+}
 
 struct Buf {
     void* p = nullptr;
@@ -32,342 +602,128 @@ int hfail(const FitCtx& c, hipError_t e, const char* what) {
     if (c.err) *c.err = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
 }
-#define FCHK(expr)                                   \
-    do {                                             \
-        hipError_t _e = (expr);                      \
+#define FCHK(expr)                                        \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
         if (_e != hipSuccess) return hfail(c, _e, #expr); \
     } while (0)
 
-// device-resident problem of one batch of evaluations
-struct FitDev {
-    Buf x, om, dev, f, g, work, act;
-    int nfit = 0, nx = 0, nw = 0;
-};
+int argfail(const FitCtx& c, const char* what) {
+    if (c.err) *c.err = what;
+    return MOED_EARG;
+}
 
-int fitdev_init(const FitCtx& c, FitDev& d, int nfit, int nx, int nw, const double* omega, const double* Dev) {
-    d.nfit = nfit;
-    d.nx = nx;
-    d.nw = nw;
-    FCHK(d.x.alloc((size_t)nfit * nx * sizeof(double)));
-    FCHK(d.om.alloc(nw * sizeof(double)));
-    FCHK(d.dev.alloc((size_t)2 * c.norb * nw * sizeof(double)));
-    FCHK(d.f.alloc(nfit * sizeof(double)));
-    FCHK(d.g.alloc((size_t)nfit * nx * sizeof(double)));
-    FCHK(d.work.alloc((size_t)nfit * c.norb * nw * 2 * sizeof(double)));
-    FCHK(d.act.alloc(nfit * sizeof(int32_t)));
-    FCHK(hipMemcpyAsync(d.om.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    FCHK(hipMemcpyAsync(d.dev.p, Dev, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+bool finite_all(const double* v, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// shape and finiteness checks shared by the entry points (skipping zero couplings relies on
+// finite inputs)
+int check_problem(const FitCtx& c, int nfit, int nx, const double* x, int nw, const double* omega,
+                  const double* Dev) {
+    if (nx > kFitNx || c.nbath > kFitMaxBath || c.norb > kMaxOrb) return argfail(c, "fit: nx or Nbath too large");
+    if (!finite_all(omega, nw) || !finite_all(Dev, (size_t)2 * c.norb * nw))
+        return argfail(c, "fit: omega and Dev must be finite");
+    for (int i = 0; i < nw; ++i)
+        if (!(omega[i] > 0.0)) return argfail(c, "fit: omega must be positive");
+    if (x && !finite_all(x, (size_t)nfit * nx)) return argfail(c, "fit: x must be finite");
     return MOED_OK;
 }
 
-// evaluate: x_host[nfit*nx]; act[i] bit0 = FUNC, bit1 = dfunc
-int fitdev_eval(const FitCtx& c, FitDev& d, const double* xh, const int32_t* acth, double* fh, double* gh) {
-    FCHK(hipMemcpyAsync(d.x.p, xh, (size_t)d.nfit * d.nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    FCHK(hipMemcpyAsync(d.act.p, acth, d.nfit * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
-    FitArgs a{d.x.as<double>(), d.nfit, d.nx, c.norb, c.nbath, d.om.as<double>(), d.dev.as<double>(), d.nw,
-              c.compat, d.f.as<double>(), d.g.as<double>(), d.work.as<double>(), d.act.as<int32_t>()};
-    FCHK(launch_fit(a, c.stream));
-    if (fh) FCHK(hipMemcpyAsync(fh, d.f.p, d.nfit * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    if (gh) FCHK(hipMemcpyAsync(gh, d.g.p, (size_t)d.nfit * d.nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    FCHK(hipStreamSynchronize(c.stream));
-    return MOED_OK;
-}
-
-// ------------------------------------------------------------------ coroutine plumbing
-
-struct FitTask {
-    struct promise_type {
-        int rc = 0;
-        FitTask get_return_object() { return FitTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
-        std::suspend_always initial_suspend() noexcept { return {}; }
-        std::suspend_always final_suspend() noexcept { return {}; }
-        void return_value(int r) { rc = r; }
-        void unhandled_exception() { std::terminate(); }
-    };
-    std::coroutine_handle<promise_type> h;
-    explicit FitTask(std::coroutine_handle<promise_type> hh) : h(hh) {}
-    FitTask(FitTask&& o) noexcept : h(o.h) { o.h = nullptr; }
-    FitTask(const FitTask&) = delete;
-    ~FitTask() {
-        if (h) h.destroy();
-    }
-};
-
-// one fit's pending evaluation request and its answer
-struct FitSlot {
-    const double* x = nullptr;  // point to evaluate (lives in the coroutine frame)
-    double* gout = nullptr;     // gradient destination, or null for FUNC only
-    double f = 0.0;
-    bool pending = false;
-    int iter = 0;
-    double fret = 0.0;
-};
-
-struct Eval {
-    FitSlot* s;
-    const double* x;
-    double* gout;
-    bool await_ready() const noexcept { return false; }
-    void await_suspend(std::coroutine_handle<>) noexcept {
-        s->x = x;
-        s->gout = gout;
-        s->pending = true;
-    }
-    double await_resume() const noexcept { return s->f; }
-};
-
-// frprmn + linmin + mnbrak + brent + f1dim (frprmn.f90:1-310) for one fit.  The constants
-// that the reference writes as REAL*4 literals keep their single-precision values.
-FitTask frprmn_task(FitSlot* S, double* p, int n, double ftol) {
-    const int ITMAX = 20000, BITMAX = 100;
-    const double EPS = 1.e-10;
-    const double LTOL = (double)1.e-8f, CGOLD = (double).3819660f, ZEPS = (double)1.0e-10f;
-    const double GOLD = (double)1.618034f, GLIMIT = 100.0, TINY = (double)1.e-20f;
-    std::vector<double> g(n), h(n), xi(n), pcom(n), xicom(n), xt(n);
-    auto fsign = [](double a, double b) { return b >= 0.0 ? std::fabs(a) : -std::fabs(a); };
-#define F1DIM(res, xv)                                                     \
-    do {                                                                   \
-        const double _xv = (xv);                                           \
-        for (int _j = 0; _j < n; ++_j) xt[_j] = pcom[_j] + _xv * xicom[_j]; \
-        res = co_await Eval{S, xt.data(), nullptr};                        \
-    } while (0)
-
-    double fp = co_await Eval{S, p, nullptr};
-    (void)co_await Eval{S, p, xi.data()};
-    for (int j = 0; j < n; ++j) {
-        g[j] = -xi[j];
-        h[j] = g[j];
-        xi[j] = h[j];
-    }
-    for (int its = 1; its <= ITMAX; ++its) {
-        S->iter = its;
-        // ---- linmin (frprmn.f90:86-117)
-        for (int j = 0; j < n; ++j) {
-            pcom[j] = p[j];
-            xicom[j] = xi[j];
-        }
-        double ax = 0., bx = 1., cx, fa, fb, fc, fu, u, ulim, q, r, dum;
-        // ---- mnbrak (frprmn.f90:236-310)
-        F1DIM(fa, ax);
-        F1DIM(fb, bx);
-        if (fb > fa) {
-            dum = ax;
-            ax = bx;
-            bx = dum;
-            dum = fb;
-            fb = fa;
-            fa = dum;
-        }
-        cx = bx + GOLD * (bx - ax);
-        F1DIM(fc, cx);
-        while (fb >= fc) {
-            r = (bx - ax) * (fb - fc);
-            q = (bx - cx) * (fb - fa);
-            u = bx - ((bx - cx) * q - (bx - ax) * r) / (2. * fsign(std::fmax(std::fabs(q - r), TINY), q - r));
-            ulim = bx + GLIMIT * (cx - bx);
-            bool ret = false;
-            if ((bx - u) * (u - cx) > 0.) {
-                F1DIM(fu, u);
-                if (fu < fc) {
-                    ax = bx;
-                    fa = fb;
-                    bx = u;
-                    fb = fu;
-                    ret = true;
-                } else if (fu > fb) {
-                    cx = u;
-                    fc = fu;
-                    ret = true;
-                } else {
-                    u = cx + GOLD * (cx - bx);
-                    F1DIM(fu, u);
-                }
-            } else if ((cx - u) * (u - ulim) > 0.) {
-                F1DIM(fu, u);
-                if (fu < fc) {
-                    bx = cx;
-                    cx = u;
-                    u = cx + GOLD * (cx - bx);
-                    fb = fc;
-                    fc = fu;
-                    F1DIM(fu, u);
-                }
-            } else if ((u - ulim) * (ulim - cx) >= 0.) {
-                u = ulim;
-                F1DIM(fu, u);
-            } else {
-                u = cx + GOLD * (cx - bx);
-                F1DIM(fu, u);
-            }
-            if (ret) break;
-            ax = bx;
-            bx = cx;
-            cx = u;
-            fa = fb;
-            fb = fc;
-            fc = fu;
-        }
-        // ---- brent (frprmn.f90:119-210) on (ax, bx, cx)
-        double a = std::fmin(ax, cx), b = std::fmax(ax, cx);
-        double v = bx, w = v, x = v, e = 0., d = 0.0, etemp, fv, fw, fx, pp, qq, rr, tol1, tol2, xm;
-        F1DIM(fx, x);
-        fv = fx;
-        fw = fx;
-        bool found = false;
-        double xmin = 0.0, fret = 0.0;
-        for (int iter = 1; iter <= BITMAX; ++iter) {
-            xm = 0.5 * (a + b);
-            tol1 = LTOL * std::fabs(x) + ZEPS;
-            tol2 = 2. * tol1;
-            if (std::fabs(x - xm) <= (tol2 - .5 * (b - a))) {
-                xmin = x;
-                fret = fx;
-                found = true;
-                break;
-            }
-            bool golden = true;
-            if (std::fabs(e) > tol1) {
-                rr = (x - w) * (fx - fv);
-                qq = (x - v) * (fx - fw);
-                pp = (x - v) * qq - (x - w) * rr;
-                qq = 2. * (qq - rr);
-                if (qq > 0.) pp = -pp;
-                qq = std::fabs(qq);
-                etemp = e;
-                e = d;
-                if (!(std::fabs(pp) >= std::fabs(.5 * qq * etemp) || pp <= qq * (a - x) || pp >= qq * (b - x))) {
-                    d = pp / qq;
-                    u = x + d;
-                    if (u - a < tol2 || b - u < tol2) d = fsign(tol1, xm - x);
-                    golden = false;
-                }
-            }
-            if (golden) {
-                e = (x >= xm) ? a - x : b - x;
-                d = CGOLD * e;
-            }
-            u = (std::fabs(d) >= tol1) ? x + d : x + fsign(tol1, d);
-            F1DIM(fu, u);
-            if (fu <= fx) {
-                if (u >= x)
-                    a = x;
-                else
-                    b = x;
-                v = w;
-                fv = fw;
-                w = x;
-                fw = fx;
-                x = u;
-                fx = fu;
-            } else {
-                if (u < x)
-                    a = u;
-                else
-                    b = u;
-                if (fu <= fw || w == x) {
-                    v = w;
-                    fv = fw;
-                    w = u;
-                    fw = fu;
-                } else if (fu <= fv || v == x || v == w) {
-                    v = u;
-                    fv = fu;
-                }
-            }
-        }
-        if (!found) co_return MOED_ECONV;  // 'brent exceed maximum iterations'
-        for (int j = 0; j < n; ++j) {
-            xi[j] = xmin * xi[j];
-            p[j] = p[j] + xi[j];
-        }
-        S->fret = fret;
-        // ---- end linmin
-        if (2.0 * std::fabs(fret - fp) <= ftol * (std::fabs(fret) + std::fabs(fp) + EPS)) co_return MOED_OK;
-        fp = fret;
-        (void)co_await Eval{S, p, xi.data()};
-        double gg = 0.0, dgg = 0.0;
-        for (int j = 0; j < n; ++j) {
-            gg = gg + g[j] * g[j];
-            dgg = dgg + (xi[j] + g[j]) * xi[j];
-        }
-        if (gg == 0.0) co_return MOED_OK;
-        const double gam = dgg / gg;
-        for (int j = 0; j < n; ++j) {
-            g[j] = -xi[j];
-            h[j] = g[j] + gam * h[j];
-            xi[j] = h[j];
-        }
-    }
-#undef F1DIM
-    co_return MOED_ECONV;  // 'frprmn maximum iterations exceeded!'
+FitProb make_prob(const FitCtx& c, int nx, int nw, const double* om, const double* dev) {
+    return FitProb{nx, c.norb, c.nbath, nw, c.compat, c.tree, om, dev};
 }
 
 }  // namespace
 
 int fit_eval(const FitCtx& c, int nfit, int nx, const double* x, int nw, const double* omega, const double* Dev,
              double* f, double* g) {
-    FitDev d;
-    int rc = fitdev_init(c, d, nfit, nx, nw, omega, Dev);
+    int rc = check_problem(c, nfit, nx, x, nw, omega, Dev);
     if (rc) return rc;
-    std::vector<int32_t> act(nfit, (f ? 1 : 0) | (g ? 2 : 0));
-    return fitdev_eval(c, d, x, act.data(), f, g);
+    Buf dx, dom, ddev, df, dg;
+    FCHK(dx.alloc((size_t)nfit * nx * sizeof(double)));
+    FCHK(dom.alloc(nw * sizeof(double)));
+    FCHK(ddev.alloc((size_t)2 * c.norb * nw * sizeof(double)));
+    FCHK(df.alloc(nfit * sizeof(double)));
+    FCHK(dg.alloc((size_t)nfit * nx * sizeof(double)));
+    FCHK(hipMemcpyAsync(dx.p, x, (size_t)nfit * nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemcpyAsync(dom.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemcpyAsync(ddev.p, Dev, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FitEvalArgs a{make_prob(c, nx, nw, dom.as<double>(), ddev.as<double>()), dx.as<double>(),
+                  f ? df.as<double>() : nullptr, g ? dg.as<double>() : nullptr};
+    hipLaunchKernelGGL(fit_eval_kernel, dim3(nfit), dim3(kFitThreads), 0, c.stream, a);
+    FCHK(hipGetLastError());
+    if (f) FCHK(hipMemcpyAsync(f, df.p, nfit * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (g) FCHK(hipMemcpyAsync(g, dg.p, (size_t)nfit * nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipStreamSynchronize(c.stream));
+    return MOED_OK;
 }
 
 int fit_delta(const FitCtx& c, int nx, const double* x, int nw, const double* omega, double* Dev) {
     // the residual pass with a zero target returns Delta itself
-    FitDev d;
-    std::vector<double> zero((size_t)2 * c.norb * nw, 0.0);
-    int rc = fitdev_init(c, d, 1, nx, nw, omega, zero.data());
-    if (rc) return rc;
-    FCHK(hipMemcpyAsync(d.x.p, x, nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    FitArgs a{d.x.as<double>(), 1, nx, c.norb, c.nbath, d.om.as<double>(), d.dev.as<double>(), nw, c.compat,
-              nullptr, nullptr, d.work.as<double>(), nullptr};
-    FCHK(launch_fit(a, c.stream));
-    FCHK(hipMemcpyAsync(Dev, d.work.p, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    Buf dx, dom, dzero, dout;
+    FCHK(dx.alloc(nx * sizeof(double)));
+    FCHK(dom.alloc(nw * sizeof(double)));
+    FCHK(dzero.alloc((size_t)2 * c.norb * nw * sizeof(double)));
+    FCHK(dout.alloc((size_t)2 * c.norb * nw * sizeof(double)));
+    FCHK(hipMemcpyAsync(dx.p, x, nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemcpyAsync(dom.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemsetAsync(dzero.p, 0, (size_t)2 * c.norb * nw * sizeof(double), c.stream));
+    FitArgs a{dx.as<double>(), nx, c.norb, c.nbath, dom.as<double>(), dzero.as<double>(), nw, c.compat,
+              dout.as<double>()};
+    FCHK(launch_fit_residual(a, c.stream));
+    FCHK(hipMemcpyAsync(Dev, dout.p, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     FCHK(hipStreamSynchronize(c.stream));
     return MOED_OK;
 }
 
 int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
                  double ftol, double* fret, int* iter, int* status) {
-    FitDev d;
-    int rc = fitdev_init(c, d, nfit, nx, nw, omega, Dev);
+    int rc = check_problem(c, nfit, nx, x, nw, omega, Dev);
     if (rc) return rc;
-    std::vector<FitSlot> slots(nfit);
-    std::vector<FitTask> tasks;
-    tasks.reserve(nfit);
-    for (int i = 0; i < nfit; ++i) tasks.push_back(frprmn_task(&slots[i], x + (size_t)i * nx, nx, ftol));
-    std::vector<double> xb((size_t)nfit * nx, 0.0), fb(nfit), gb((size_t)nfit * nx);
-    std::vector<int32_t> act(nfit);
-    for (int i = 0; i < nfit; ++i) tasks[i].h.resume();  // run to the first evaluation
-    while (true) {
-        bool any = false, anyg = false;
-        for (int i = 0; i < nfit; ++i) {
-            act[i] = 0;
-            if (tasks[i].h.done() || !slots[i].pending) continue;
-            any = true;
-            act[i] = slots[i].gout ? 2 : 1;
-            anyg = anyg || slots[i].gout;
-            std::copy(slots[i].x, slots[i].x + nx, xb.begin() + (size_t)i * nx);
-        }
-        if (!any) break;
-        rc = fitdev_eval(c, d, xb.data(), act.data(), fb.data(), anyg ? gb.data() : nullptr);
-        if (rc) return rc;
-        for (int i = 0; i < nfit; ++i) {
-            if (!act[i]) continue;
-            FitSlot& s = slots[i];
-            s.pending = false;
-            if (s.gout)
-                std::copy(gb.begin() + (size_t)i * nx, gb.begin() + (size_t)(i + 1) * nx, s.gout);
-            else
-                s.f = fb[i];
-            tasks[i].h.resume();
-        }
+    Buf dx, dom, ddev, dst, dist, dact;
+    FCHK(dx.alloc((size_t)nfit * nx * sizeof(double)));
+    FCHK(dom.alloc(nw * sizeof(double)));
+    FCHK(ddev.alloc((size_t)2 * c.norb * nw * sizeof(double)));
+    FCHK(dst.alloc((size_t)nfit * kStateD * sizeof(double)));
+    FCHK(dist.alloc((size_t)nfit * kStateI * sizeof(int32_t)));
+    FCHK(dact.alloc(sizeof(int32_t)));
+    FCHK(hipMemcpyAsync(dx.p, x, (size_t)nfit * nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemcpyAsync(dom.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemcpyAsync(ddev.p, Dev, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    FCHK(hipMemsetAsync(dst.p, 0, (size_t)nfit * kStateD * sizeof(double), c.stream));
+    FCHK(hipMemsetAsync(dist.p, 0, (size_t)nfit * kStateI * sizeof(int32_t), c.stream));
+    const int budget = c.budget > 0 ? c.budget : (c.tree ? 1024 : 128);
+    FitCgArgs a{make_prob(c, nx, nw, dom.as<double>(), ddev.as<double>()), nfit, budget, ftol, dx.as<double>(),
+                dst.as<double>(), dist.as<int32_t>(), dact.as<int32_t>()};
+    // every launch ends each running fit after `budget` more iterations; kItmax bounds the loop
+    for (int launch = 0; launch <= kItmax; ++launch) {
+        FCHK(hipMemsetAsync(dact.p, 0, sizeof(int32_t), c.stream));
+        hipLaunchKernelGGL(fit_cg_kernel, dim3(nfit), dim3(kFitThreads), 0, c.stream, a);
+        FCHK(hipGetLastError());
+        int32_t active = 0;
+        FCHK(hipMemcpyAsync(&active, dact.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+        FCHK(hipStreamSynchronize(c.stream));
+        if (active == 0) break;
     }
+    std::vector<double> stv((size_t)nfit * kStateD);
+    std::vector<int32_t> isv((size_t)nfit * kStateI);
+    FCHK(hipMemcpyAsync(x, dx.p, (size_t)nfit * nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipMemcpyAsync(stv.data(), dst.p, stv.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipMemcpyAsync(isv.data(), dist.p, isv.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipStreamSynchronize(c.stream));
     for (int i = 0; i < nfit; ++i) {
-        if (fret) fret[i] = slots[i].fret;
-        if (iter) iter[i] = slots[i].iter;
-        if (status) status[i] = tasks[i].h.promise().rc;
+        const int32_t* s = isv.data() + (size_t)i * kStateI;
+        if (s[0] != kPhaseDone) {
+            if (c.err) *c.err = "fit: a fit did not finish";
+            return MOED_ECONV;
+        }
+        if (fret) fret[i] = stv[(size_t)i * kStateD + 3 * kFitNx + 1];
+        if (iter) iter[i] = s[1];
+        if (status) status[i] = s[2];
+        if (c.nfev) c.nfev[i] = s[3];
     }
     return MOED_OK;
 }

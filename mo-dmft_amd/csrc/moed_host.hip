@@ -127,6 +127,8 @@ struct moed_handle {
     std::vector<Slot> slots;
     DevBuf d_slotc;
     int hv_threads = 0, hv_strip = 0;
+    int fit_budget = 0;            // CG iterations per fit and launch of moed_minimize (0: default)
+    std::vector<int32_t> fit_nfev;  // function evaluations of each fit of the last moed_minimize
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
@@ -895,7 +897,8 @@ int green_diag(moed_handle* h, const Sector& s, const double* psi, double E, dou
 }  // namespace
 
 static FitCtx fit_ctx(moed_handle* h) {
-    return FitCtx{h->stream, h->norb, h->nbath, (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1, &h->err};
+    return FitCtx{h->stream, h->norb, h->nbath, (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1,
+                  (h->flags & MOED_F_FIT_TREE) ? 1 : 0, h->fit_budget, nullptr, &h->err};
 }
 
 int fit_delta_host(moed_handle* h, int nx, const double* x, int nw, const double* omega, double* Dev) {
@@ -912,7 +915,10 @@ int fit_eval_host(moed_handle* h, int nfit, int nx, const double* x, int nw, con
 int fit_minimize_host(moed_handle* h, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
                       double ftol, double* fret, int* iter, int* status) {
     (void)hipSetDevice(h->device);
-    return fit_minimize(fit_ctx(h), nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
+    FitCtx c = fit_ctx(h);
+    h->fit_nfev.assign(nfit, 0);
+    c.nfev = h->fit_nfev.data();
+    return fit_minimize(c, nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
 }
 
 // =================================================================== C ABI
@@ -1380,6 +1386,20 @@ int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x, 
     if (!h || !x || n < 0) return MOED_EARG;
     (void)hipSetDevice(h->device);
     HIPCHK(launch_fill_start(seed, n, x, stream ? (hipStream_t)stream : h->stream));
+    return MOED_OK;
+}
+
+int moed_set_fit_launch(moed_handle* h, int32_t iterations) {
+    if (!h) return MOED_EARG;
+    if (iterations < 0) return fail(h, MOED_EARG, "iterations must be >= 0");
+    h->fit_budget = iterations;
+    return MOED_OK;
+}
+
+int moed_fit_nfev(moed_handle* h, int32_t nfit, int32_t* nfev) {
+    if (!h || !nfev || nfit < 0) return MOED_EARG;
+    if ((size_t)nfit > h->fit_nfev.size()) return fail(h, MOED_ESTATE, "fit_nfev: fewer fits in the last moed_minimize");
+    std::copy(h->fit_nfev.begin(), h->fit_nfev.begin() + nfit, nfev);
     return MOED_OK;
 }
 

@@ -213,18 +213,15 @@ struct DeltaArgs {
 };
 hipError_t launch_delta_new(const DeltaArgs& a, hipStream_t s);
 
-// batched FUNC + dfunc (func.f90:1-142) in the reference's summation order
+// Delta(i w) - Dev of one parameter vector (func.f90:19-29), used by calc_dev with a zero target
 struct FitArgs {
-    const double* x;  // [nfit][nx]
-    int32_t nfit, nx, norb, nbath;
+    const double* x;  // [nx], packed as ev_to_x
+    int32_t nx, norb, nbath;
     const double* omega;
     const double* dev;  // interleaved complex (norb, nw) column-major
     int32_t nw, compat;
-    double* f;      // [nfit] or nullptr
-    double* g;      // [nfit][nx] or nullptr
-    double* work;   // scratch [nfit][norb][nw] complex residuals
-    const int32_t* active;  // optional [nfit]: bit 0 evaluate FUNC, bit 1 evaluate dfunc
+    double* work;       // out: interleaved complex (norb, nw)
 };
-hipError_t launch_fit(const FitArgs& a, hipStream_t s);
+hipError_t launch_fit_residual(const FitArgs& a, hipStream_t s);
 
 }  // namespace moed

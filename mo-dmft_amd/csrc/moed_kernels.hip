@@ -6,69 +6,11 @@
 
 #include <cmath>
 
+#include "moed_cplx.h"
 #include "moed_device.h"
 #include "moed_internal.h"
 
 namespace moed {
-
-// ---------------------------------------------------------------- helpers
-
-// Complex arithmetic as the reference build evaluates it: textbook division, a real operand
-// promoted to (x, 0), no contraction (every product and sum rounded on its own).
-struct cxd {
-    double re, im;
-};
-__device__ __forceinline__ cxd cx_div(cxd a, cxd b) {
-#pragma clang fp contract(off)
-    const double den = b.re * b.re + b.im * b.im;
-    return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
-}
-__device__ __forceinline__ cxd cx_rdiv(double r, cxd b) { return cx_div({r, 0.0}, b); }
-__device__ __forceinline__ cxd cx_mul(cxd a, cxd b) {
-#pragma clang fp contract(off)
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-
-// |z| as sqrt(ax^2 + ay^2) followed by one correction step without FMA: C. F. Borges,
-// "An Improved Algorithm for hypot(a,b)", ACM TOMS 47 (2021), the fma-free corrected form.
-// Range guards keep ax^2 + ay^2 finite and normal; for the inputs of the bath fit the plain
-// path is taken.
-__device__ __forceinline__ double hypot_corrected(double x, double y) {
-#pragma clang fp contract(off)
-    x = fabs(x);
-    y = fabs(y);
-    const double ax = x < y ? y : x;
-    const double ay = x < y ? x : y;
-    const double EPS = 0x1p-54, SCALE = 0x1p-600;
-    auto kernel = [](double a, double b) {
-#pragma clang fp contract(off)
-        double h = sqrt(a * a + b * b);
-        double t1, t2;
-        if (h <= 2.0 * b) {
-            const double delta = h - b;
-            t1 = a * (2.0 * delta - a);
-            t2 = (delta - 2.0 * (a - b)) * delta;
-        } else {
-            const double delta = h - a;
-            t1 = 2.0 * delta * (a - 2.0 * b);
-            t2 = (4.0 * delta - b) * b + delta * delta;
-        }
-        h -= (t1 + t2) / (2.0 * h);
-        return h;
-    };
-    if (ax > 0x1p+511) {
-        if (ay <= ax * EPS) return ax + ay;
-        return kernel(ax * SCALE, ay * SCALE) / SCALE;
-    }
-    if (ay < 0x1p-511) {
-        if (ax >= ay / EPS) return ax + ay;
-        return kernel(ax / SCALE, ay / SCALE) * SCALE;
-    }
-    if (ay <= ax * EPS) return ax + ay;
-    return kernel(ax, ay);
-}
-
-__device__ __forceinline__ double round_f32(double v) { return (double)(float)v; }
 
 // ---------------------------------------------------------------- apply_c
 
@@ -344,87 +286,22 @@ __device__ __forceinline__ cxd fit_delta(const double* x, int nx, int norb, int 
     return {re + x[nx - norb + o], im};
 }
 
-// pass 1: residuals r_o(w) = Delta_o - Dev_o for every (fit, w), one thread each
+// r_o(w) = Delta_o(i w) - Dev_o(w) for one parameter vector, one thread per frequency; with a zero
+// target this is calc_dev's Delta (ed_green.F90:260-273)
 __global__ void fit_residual_kernel(FitArgs a) {
     const int iw = blockIdx.x * blockDim.x + threadIdx.x;
-    const int f = blockIdx.y;
     if (iw >= a.nw) return;
-    if (a.active && !a.active[f]) return;
-    const double* x = a.x + (size_t)f * a.nx;
     const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
     for (int o = 0; o < a.norb; ++o) {
-        const cxd d = fit_delta(x, a.nx, a.norb, a.nbath, o, z);
+        const cxd d = fit_delta(a.x, a.nx, a.norb, a.nbath, o, z);
         const size_t io = (size_t)o + (size_t)a.norb * iw;
-        double* r = a.work + ((size_t)f * a.norb * a.nw + io) * 2;
-        r[0] = d.re - a.dev[2 * io];
-        r[1] = d.im - a.dev[2 * io + 1];
+        a.work[2 * io] = d.re - a.dev[2 * io];
+        a.work[2 * io + 1] = d.im - a.dev[2 * io + 1];
     }
 }
 
-// pass 2: one thread per (fit, component) summing sequentially in the reference's loop order.
-// component 0: FUNC (func.f90:30-42); 1..nbath: d/de_k (func.f90:89-101); then d/dV (:103-116),
-// then d/def (:118-124); all divided as func.f90:131.
-__global__ void fit_sum_kernel(FitArgs a) {
-#pragma clang fp contract(off)
-    const int f = blockIdx.y;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > a.nx) return;
-    if (a.active && !(a.active[f] & (c == 0 ? 1 : 2))) return;
-    if (c == 0 && !a.f) return;
-    if (c > 0 && !a.g) return;
-    const int norb = a.norb, nbath = a.nbath, nw = a.nw, nx = a.nx;
-    const double* x = a.x + (size_t)f * nx;
-    const double* R = a.work + (size_t)f * norb * nw * 2;
-    double s = 0.0;
-    if (c == 0) {
-        for (int iw = 0; iw < nw; ++iw)
-            for (int o = 0; o < norb; ++o) {
-                const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
-                const double h = hypot_corrected(r[0], r[1]);
-                s = s + h * h / a.omega[iw];
-            }
-        a.f[f] = s / (double)(float)(norb * norb * nw);
-        return;
-    }
-    const int p = c - 1;
-    if (p < nbath) {
-        const int k = p;
-        for (int o = 0; o < norb; ++o) {
-            const double v = x[nbath + o * nbath + k];
-            for (int iw = 0; iw < nw; ++iw) {
-                const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
-                const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
-                const cxd den = {z.re - x[k], z.im};
-                const cxd cr = {r[0], -r[1]};
-                const cxd t = cx_div(cx_div(cx_mul(cx_mul(cr, {v, 0.0}), {v, 0.0}), den), den);
-                s = s + 2.0 * t.re / a.omega[iw];
-            }
-        }
-    } else if (p < nx - norb) {
-        const int o = (p - nbath) / nbath, k = (p - nbath) % nbath;
-        const double v = x[p];
-        for (int iw = 0; iw < nw; ++iw) {
-            const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
-            const cxd z = {0.0, a.compat ? round_f32(a.omega[iw]) : a.omega[iw]};
-            const cxd den = {z.re - x[k], z.im};
-            const cxd cr = {r[0], -r[1]};
-            const cxd q = cx_div(cx_mul(cr, {v, 0.0}), den);
-            const double tre = q.re + q.re;
-            s = s + 2.0 * tre / a.omega[iw];
-        }
-    } else {
-        const int o = p - (nx - norb);
-        for (int iw = 0; iw < nw; ++iw) {
-            const double* r = R + ((size_t)o + (size_t)norb * iw) * 2;
-            s = s + 2.0 * r[0] / a.omega[iw];
-        }
-    }
-    a.g[(size_t)f * nx + p] = s / (double)(float)(norb * norb) / (double)(float)nw;
-}
-
-hipError_t launch_fit(const FitArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(fit_residual_kernel, dim3((a.nw + 127) / 128, a.nfit), dim3(128), 0, s, a);
-    hipLaunchKernelGGL(fit_sum_kernel, dim3((a.nx + 1 + 63) / 64, a.nfit), dim3(64), 0, s, a);
+hipError_t launch_fit_residual(const FitArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(fit_residual_kernel, dim3((a.nw + 127) / 128), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

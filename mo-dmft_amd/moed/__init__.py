@@ -30,6 +30,7 @@ MOED_F_EXACT_CMPLX = 1
 MOED_F_GF_REORTH = 2
 MOED_F_NO_TUNING = 4
 MOED_F_BATCHED = 8
+MOED_F_FIT_TREE = 16
 
 _ERR = {-1: "EARG", -2: "ESECTOR", -3: "EZERO", -4: "EHIP", -5: "ENOMEM", -6: "ESTATE", -7: "ECONV"}
 
@@ -58,6 +59,7 @@ EXPORTS = [
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
     "moed_fill_start_device", "moed_set_hv_shape", "moed_stream", "moed_band_tb", "moed_delta_new",
+    "moed_set_fit_launch", "moed_fit_nfev",
 ]
 
 _lib = None
@@ -98,6 +100,8 @@ def lib():
         L.moed_omega_grid.argtypes = [ct.c_int32, ct.c_double, _dp]
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
+        L.moed_set_fit_launch.argtypes = [ct.c_void_p, ct.c_int32]
+        L.moed_fit_nfev.argtypes = [ct.c_void_p, ct.c_int32, _ip]
         L.moed_band_tb.argtypes = [ct.c_int32, ct.c_int32, _dp]
         L.moed_delta_new.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_double, _dp, _dp, _dp, _dp, _dp]
         L.moed_stream.argtypes = [ct.c_void_p]
@@ -347,6 +351,16 @@ class EDSolver:
 
     def set_hv_shape(self, threads=0, strip=0):
         self._check(lib().moed_set_hv_shape(self.h, threads, strip))
+
+    def set_fit_launch(self, iterations=0):
+        """CG iterations per fit and kernel launch inside minimization (0: default)"""
+        self._check(lib().moed_set_fit_launch(self.h, iterations))
+
+    def fit_nfev(self, nfit):
+        """FUNC evaluations of each fit of the last minimization"""
+        n = np.zeros(nfit, np.int32)
+        self._check(lib().moed_fit_nfev(self.h, nfit, n.ctypes.data_as(_ip)))
+        return n
 
     @property
     def stream(self):

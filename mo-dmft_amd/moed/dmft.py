@@ -38,8 +38,10 @@ def start_points(x0, nfit, seed, spread):
     return X
 
 
-def dmft_loop(solver, hk, mu, omega, niter, nfit=1, seed=1, spread=0.1, ftol=1.0e-7, scf_tol=1.0e-4):
-    """Run up to ``niter`` iterations; returns the per-iteration records and the final bath."""
+def dmft_loop(solver, hk, mu, omega, niter, nfit=1, seed=1, spread=0.1, ftol=1.0e-7, scf_tol=1.0e-4,
+              on_iter=None):
+    """Run up to ``niter`` iterations; returns the per-iteration records and the final bath.
+    ``on_iter(record)`` is called after each iteration (progress reporting)."""
     norb, nbath = solver.norb, solver.nbath
     ek = solver._ek.copy()
     vk = solver._vk.reshape(norb, nbath, order="F").copy()
@@ -68,6 +70,8 @@ def dmft_loop(solver, hk, mu, omega, niter, nfit=1, seed=1, spread=0.1, ftol=1.0
         out.append(dict(iteration=it, E0=kept[0][0], nocc=nocc.tolist(), fret=float(fret[best]), best=best,
                         fit_iters=int(iters[best]), scf_diff=tol, t_solve=t1 - t0, t_green=t2 - t1,
                         t_delta=t3 - t2, t_fit=t4 - t3, t_total=t5 - t0))
+        if on_iter is not None:
+            on_iter(out[-1])
         if tol < scf_tol:
             break
     return out, (ef, ek, vk)

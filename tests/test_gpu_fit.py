@@ -1,13 +1,14 @@
 """GPU parity of the batched bath fit: FUNC/dfunc (func.f90) in the reference's operation order,
-calc_dev, and the CG minimiser (frprmn.f90) against the reference build's trajectory."""
+calc_dev, and the CG minimiser (frprmn.f90) against the reference build's trajectory; the
+tree-order mode against the oracle within rounding."""
 import os
 
 import numpy as np
 import pytest
 
 import oracle
-from moed import EDSolver
-from moed.model import omega_grid
+from moed import EDSolver, MOED_F_FIT_TREE
+from moed.model import config, omega_grid
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -54,6 +55,9 @@ def test_minimization_reproduces_reference_trajectory(fit):
     assert st[0] == 0
     assert np.max(np.abs(x[0] - z["xmin"])) <= 1e-8
     assert abs(fret[0] - float(z["fret"])) <= 1e-8 * max(1e-12, abs(float(z["fret"]))) + 1e-20
+    rc, xo, fo, ito, nfo = oracle.frprmn(z["x0"], z["omega"], z["Dev"], norb, nbath)
+    assert np.array_equal(x[0], xo) and fret[0] == fo and it[0] == ito
+    assert s.fit_nfev(1)[0] == nfo  # the same line searches, evaluation for evaluation
 
 
 def test_minimization_batched_independent_starts(fit):
@@ -66,6 +70,103 @@ def test_minimization_batched_independent_starts(fit):
         rc, xo, fo, ito, _ = oracle.frprmn(X[i], z["omega"], z["Dev"], norb, nbath)
         assert st[i] == rc == 0
         assert it[i] == ito and np.max(np.abs(x[i] - xo)) <= 1e-8
+
+
+def test_results_do_not_depend_on_the_launch_budget(fit):
+    """the minimiser suspends at CG iteration boundaries between launches: one iteration per
+    launch and the default give bit-identical fits"""
+    z, s, norb, nbath = fit
+    rng = np.random.default_rng(5)
+    X = z["x0"][None, :] + 0.02 * rng.standard_normal((3, len(z["x0"]))) * (z["x0"] != 0)
+    out = []
+    for budget in (0, 1, 7):
+        s.set_fit_launch(budget)
+        out.append(s.minimization(X, z["omega"], z["Dev"]) + (s.fit_nfev(3),))
+    s.set_fit_launch(0)
+    for r in out[1:]:
+        for a, b in zip(r, out[0]):
+            assert np.array_equal(a, b)
+
+
+@pytest.fixture(scope="module")
+def fit_tree():
+    z = np.load(os.path.join(GOLD, "ref_fit.npz"))
+    norb, nbath = int(z["norb"]), int(z["nbath"])
+    s = EDSolver(norb, nbath, 0.0, 0.0, np.zeros(norb + nbath), np.zeros((norb, nbath)), flags=MOED_F_FIT_TREE)
+    yield z, s, norb, nbath
+    s.close()
+
+
+def test_tree_order_func_dfunc_within_rounding(fit_tree):
+    """tree-order sums: chi^2 and gradient agree with the oracle to summation rounding"""
+    z, s, norb, nbath = fit_tree
+    rng = np.random.default_rng(2)
+    X = z["x0"][None, :] + 0.05 * rng.standard_normal((9, len(z["x0"]))) * (z["x0"] != 0)
+    f, g = s.func_dfunc(X, z["omega"], z["Dev"])
+    for i in range(len(X)):
+        fo = oracle.func(X[i], z["omega"], z["Dev"], norb, nbath)
+        go = oracle.dfunc(X[i], z["omega"], z["Dev"], norb, nbath)
+        assert abs(f[i] - fo) <= 1e-13 * abs(fo)
+        assert np.max(np.abs(g[i] - go)) <= 1e-12 * np.abs(go).max()
+
+
+def test_tree_order_fit_converges_to_the_reference_minimum(fit_tree):
+    z, s, norb, nbath = fit_tree
+    x, fret, it, st = s.minimization(z["x0"], z["omega"], z["Dev"])
+    fr, f0 = float(z["fret"]), float(z["f"])
+    print("tree fret", fret[0], "ref", fr, "start", f0, "iters", it[0], flush=True)
+    assert st[0] == 0
+    # the fixture's target is fitted exactly: both minima are chi^2 ~ 0 on the scale of the start
+    assert fret[0] <= 1e-8 * f0 and abs(fret[0] - fr) <= 1e-8 * f0
+
+
+def test_tree_order_fit_is_independent_of_the_batch(fit_tree):
+    z, s, norb, nbath = fit_tree
+    rng = np.random.default_rng(8)
+    X = z["x0"][None, :] + 0.02 * rng.standard_normal((5, len(z["x0"]))) * (z["x0"] != 0)
+    xb, fb, ib, sb = s.minimization(X, z["omega"], z["Dev"])
+    for i in (0, 3):
+        x1, f1, i1, s1 = s.minimization(X[i], z["omega"], z["Dev"])
+        assert np.array_equal(x1[0], xb[i]) and f1[0] == fb[i] and i1[0] == ib[i]
+
+
+def test_func_dfunc_at_the_15_site_fit_shape():
+    """Norb 3, Nbath 12 (nx 51), 4096 frequencies at beta 200: reference order bit-exact with the
+    oracle, tree order within rounding"""
+    c = config("ns15_fit")
+    norb, nbath = c.norb, c.nbath
+    om = omega_grid(c.nw, c.beta)
+    from moed import dmft
+    x0 = dmft.ev_to_x(c.ek, c.vk, c.ek[:norb], norb, nbath)
+    rng = np.random.default_rng(15)
+    X = x0[None, :] * (1.0 + 0.1 * rng.standard_normal((4, len(x0))))
+    ef, ek, vk = oracle.x_to_ev(X[3], norb, nbath)
+    Dev = oracle.calc_dev(oracle.Model(norb, nbath, 0, 0, ek, vk), ef, om)
+    ref = [(oracle.func(x, om, Dev, norb, nbath), oracle.dfunc(x, om, Dev, norb, nbath)) for x in X]
+    for flags in (0, MOED_F_FIT_TREE):
+        with EDSolver(norb, nbath, 0.0, 0.0, np.zeros(norb + nbath), np.zeros((norb, nbath)), flags=flags) as s:
+            f, g = s.func_dfunc(X, om, Dev)
+        for i in range(len(X)):
+            if flags == 0:
+                assert f[i] == ref[i][0] and np.array_equal(g[i], ref[i][1])
+            else:
+                assert abs(f[i] - ref[i][0]) <= 1e-13 * abs(ref[i][0]) + 1e-300
+                assert np.max(np.abs(g[i] - ref[i][1])) <= 1e-12 * max(np.abs(ref[i][1]).max(), 1e-300)
+
+
+def test_fit_rejects_bad_inputs(fit):
+    z, s, norb, nbath = fit
+    om = z["omega"].copy()
+    with pytest.raises(Exception):
+        s.minimization(z["x0"], -om, z["Dev"])
+    D = z["Dev"].copy()
+    D[0, 0] = np.nan
+    with pytest.raises(Exception):
+        s.minimization(z["x0"], om, D)
+    x = z["x0"].copy()
+    x[0] = np.inf
+    with pytest.raises(Exception):
+        s.func_dfunc(x, om, z["Dev"])
 
 
 def test_exact_fit_recovers_parameters():
