@@ -36,7 +36,7 @@ constexpr int kFitThreads = 1024;           // 16 waves: four per SIMD hide the 
 constexpr int kFitNx = 64;                    // >= nxsize; the reference's nmax is 60 (frprmn.f90:12)
 constexpr int kFitMaxBath = kMaxSite - 1;
 constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
-constexpr int kFitChunkF = kFitThreads - 64;   // frequencies per staged FUNC chunk (waves 1..)
+constexpr int kFitChunkF = kFitThreads / 4 * 3;  // frequencies per staged FUNC chunk (waves w % 4 != 0)
 constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
 constexpr int kItmax = 20000, kBrentItmax = 100;  // frprmn.f90:12, :126
 constexpr int kStateD = 3 * kFitNx + 2;       // per fit: xi, g, h, fp, fret
@@ -120,54 +120,133 @@ __device__ __forceinline__ cxd residual(const FitView& q, const double* xt, int 
 }
 
 // s + buf[0] + buf[1] + ... + buf[n-1], one rounding per add in this order; the LDS loads run
-// eight elements ahead of the dependent adds
+// sixteen elements ahead of the dependent adds
 __device__ __forceinline__ double chain_sum(double s, const double* buf, int n) {
 #pragma clang fp contract(off)
+    constexpr int B = 16;
     int j = 0;
-    if (n >= 16) {
-        double a[8], b[8];
+    if (n >= 2 * B) {
+        double a[B], b[B];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) a[t] = buf[t];
-        for (j = 8; j + 8 <= n; j += 8) {
+        for (int t = 0; t < B; ++t) a[t] = buf[t];
+        for (j = B; j + B <= n; j += B) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) b[t] = buf[j + t];
+            for (int t = 0; t < B; ++t) b[t] = buf[j + t];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) s = s + a[t];
+            for (int t = 0; t < B; ++t) s = s + a[t];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) a[t] = b[t];
+            for (int t = 0; t < B; ++t) a[t] = b[t];
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) s = s + a[t];
+        for (int t = 0; t < B; ++t) s = s + a[t];
     }
     for (; j < n; ++j) s = s + buf[j];
     return s;
 }
 
+// Tree order restates the terms with fewer divisions (each term then differs from the reference's
+// by a few ulps): 1/(i w - e_k) = (-e_k - i w) / (e_k^2 + w^2) with one division, 1/w_n once per
+// frequency, |r|^2 instead of hypot(r)^2.
+__device__ __forceinline__ cxd inv_den(double ek, double zi) {
+#pragma clang fp contract(off)
+    const double c = 0.0 - ek, inv = 1.0 / (c * c + zi * zi);
+    return {c * inv, -(zi * inv)};
+}
+
+// A thread's group of kGrp frequencies iw = base + tid + j kFitThreads.  The k loop runs outside
+// the frequency loop, so V(o,k) and e_k are read and tested once per k and the kGrp divisions of
+// one k are independent of each other.
+constexpr int kGrp = 4;
+struct FreqGroup {
+    double zi[kGrp], rw[kGrp];  // Im z, 1/w_n (0 for a frequency past the end)
+    int iw[kGrp];
+};
+
+__device__ __forceinline__ FreqGroup freq_group(const FitView& q, int base) {
+    FreqGroup g;
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {
+        const int iw = base + (int)threadIdx.x + j * kFitThreads;
+        const bool ok = iw < q.nw;
+        const double w = ok ? q.omega[iw] : 1.0;
+        g.iw[j] = ok ? iw : -1;
+        g.zi[j] = zim(q, w);
+        g.rw[j] = ok ? 1.0 / w : 0.0;
+    }
+    return g;
+}
+
+// r_o(w) = Delta_o(i w) - Dev_o(w) for the group (zero past the end)
+struct GroupResidual {
+    double re[kGrp], im[kGrp];
+};
+__device__ __forceinline__ GroupResidual residual_group(const FitView& q, const double* xt, int o,
+                                                        const FreqGroup& g) {
+    GroupResidual r;
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) r.re[j] = r.im[j] = 0.0;
+    for (int k = 0; k < q.nbath; ++k) {
+        const double v = xt[q.nbath + o * q.nbath + k];
+        if (v == 0.0) continue;
+        const double ek = xt[k], v2 = v * v;
+#pragma unroll
+        for (int j = 0; j < kGrp; ++j) {
+            const cxd d = inv_den(ek, g.zi[j]);
+            r.re[j] = fma(v2, d.re, r.re[j]);
+            r.im[j] = fma(v2, d.im, r.im[j]);
+        }
+    }
+    const double ef = xt[q.nx - q.norb + o];
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) {
+        const bool ok = g.iw[j] >= 0;
+        const size_t io = (size_t)o + (size_t)q.norb * (ok ? g.iw[j] : 0);
+        const double dre = q.dev[2 * io], dim = q.dev[2 * io + 1];
+        r.re[j] = ok ? (r.re[j] + ef) - dre : 0.0;
+        r.im[j] = ok ? r.im[j] - dim : 0.0;
+    }
+    return r;
+}
+
 // FUNC (func.f90:30-42): sum over (w, o) of |r|^2 / w_n, divided by float(Norb*Norb*Nw).
 // Reference order: waves 1.. stage the terms of frequency chunk c while lane 0 of wave 0 adds chunk
 // c - 1 to the running sum (double-buffered).  Tree order: per-thread partials, then block_sum.
-__device__ __noinline__ double eval_func(const FitProb qa) {
+__device__ __noinline__ double eval_func_tree(const FitProb qa) {
 #pragma clang fp contract(off)
     const FitView q = view(qa);
     const int tid = threadIdx.x;
     double s = 0.0;
-    if (uniform_i32(qa.tree)) {
-        for (int iw = tid; iw < q.nw; iw += kFitThreads) {
-            const double w = q.omega[iw], zi = zim(q, w);
+    {
+        for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
+            const FreqGroup g = freq_group(q, base);
             for (int o = 0; o < q.norb; ++o) {
-                const cxd r = residual(q, S.xt, o, iw, zi);
-                const double h = hypot_corrected(r.re, r.im);
-                s = s + h * h / w;
+                const GroupResidual r = residual_group(q, S.xt, o, g);
+#pragma unroll
+                for (int j = 0; j < kGrp; ++j) s = fma(fma(r.re[j], r.re[j], r.im[j] * r.im[j]), g.rw[j], s);
             }
         }
         s = block_sum<kFitThreads>(s, S.red);
-    } else {
+    }
+    if (tid == 0) S.bcast[0] = s / (double)(float)(q.norb * q.norb * q.nw);
+    __syncthreads();
+    return S.bcast[0];
+}
+
+__device__ __noinline__ double eval_func_ref(const FitProb qa) {
+#pragma clang fp contract(off)
+    const FitView q = view(qa);
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    {
+        // waves w % 4 != 0 stage (waves are spread over the SIMDs round-robin, so the chain's SIMD
+        // keeps no staging wave and its dependent adds do not wait behind divisions)
+        const int wid = tid >> 6, slot = (wid - 1 - (wid - 1) / 4) * 64 + (tid & 63);
         const int nch = (q.nw + kFitChunkF - 1) / kFitChunkF;
         for (int c = 0; c <= nch; ++c) {
-            if (tid >= 64 && c < nch) {
-                const int iw = c * kFitChunkF + (tid - 64);
+            if ((wid & 3) != 0 && c < nch) {
+                const int iw = c * kFitChunkF + slot;
                 if (iw < q.nw) {
-                    double* st = S.u.fstage[c & 1] + (tid - 64) * q.norb;
+                    double* st = S.u.fstage[c & 1] + slot * q.norb;
                     const double w = q.omega[iw], zi = zim(q, w);
                     for (int o = 0; o < q.norb; ++o) {
                         const cxd r = residual(q, S.xt, o, iw, zi);
@@ -186,6 +265,10 @@ __device__ __noinline__ double eval_func(const FitProb qa) {
     if (tid == 0) S.bcast[0] = s / (double)(float)(q.norb * q.norb * q.nw);
     __syncthreads();
     return S.bcast[0];
+}
+
+__device__ __forceinline__ double eval_func(const FitProb& q) {
+    return uniform_i32(q.tree) ? eval_func_tree(q) : eval_func_ref(q);
 }
 
 // gradient terms of one (w, o) (func.f90:89-124): column k = d/de_k, nbath + k = d/dV(o,k),
@@ -220,51 +303,56 @@ __device__ __forceinline__ void grad_terms(const FitView& q, const double* xt, i
 __device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
 #pragma clang fp contract(off)
     const FitView q = view(qa);
-    const int tid = threadIdx.x, nb = q.nbath;
+    const int tid = threadIdx.x, nb = q.nbath, lane = tid & 63, wid = tid >> 6;
     const double scale1 = (double)(float)(q.norb * q.norb), scale2 = (double)(float)q.nw;
-    {
-        const int lane = tid & 63, wid = tid >> 6;
-        for (int o = 0; o < q.norb; ++o) {
-            double part[kFitCols];  // columns k, kFitMaxBath + k, 2 kFitMaxBath: registers
+    for (int c = tid; c < q.nx; c += kFitThreads) S.gacc[c] = 0.0;
+    // with d = 1/(i w - e_k): d/de_k 2 Re(conj(r) V^2 d^2)/w, d/dV 4 Re(conj(r) V d)/w, d/def 2 Re(r)/w.
+    // Per (orbital, frequency group): each column is summed over the thread's frequencies, over the
+    // wave, then over the waves in index order, and added to the running total.
+    for (int o = 0; o < q.norb; ++o) {
+        for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
+            const FreqGroup g = freq_group(q, base);
+            const GroupResidual r = residual_group(q, S.xt, o, g);
+            for (int k = 0; k < nb; ++k) {
+                const double v = S.xt[nb + o * nb + k];
+                if (v == 0.0) continue;
+                const double ek = S.xt[k], v2 = v * v;
+                double pde = 0.0, pdv = 0.0;
 #pragma unroll
-            for (int c = 0; c < kFitCols; ++c) part[c] = 0.0;
-            for (int iw = tid; iw < q.nw; iw += kFitThreads) {
-                // keeps the parameter loads (LDS) inside the loop: hoisting all of them would
-                // push the register partials into scratch
-                asm volatile("" ::: "memory");
-                grad_terms(q, S.xt, o, iw, [&](int c, double t) { part[c] = part[c] + t; });
+                for (int j = 0; j < kGrp; ++j) {
+                    const cxd d = inv_den(ek, g.zi[j]);
+                    const double ure = fma(r.re[j], d.re, r.im[j] * d.im);
+                    const double uim = fma(r.re[j], d.im, -(r.im[j] * d.re));
+                    const double rw2 = 2.0 * g.rw[j];
+                    pde = fma(v2 * fma(ure, d.re, -(uim * d.im)), rw2, pde);
+                    pdv = fma(2.0 * v * ure, rw2, pdv);
+                }
+                pde = wave_sum(pde);
+                pdv = wave_sum(pdv);
+                if (lane == 0) {
+                    S.gred[wid][k] = pde;
+                    S.gred[wid][kFitMaxBath + k] = pdv;
+                }
             }
-            // fixed tree: wave sums, then the waves in index order
+            double pdef = 0.0;
 #pragma unroll
-            for (int c = 0; c < kFitCols; ++c) {
-                const int k = c < kFitMaxBath ? c : c - kFitMaxBath;
-                if (c != 2 * kFitMaxBath && k >= nb) continue;
-                const double v = wave_sum(part[c]);
-                if (lane == 0) S.gred[wid][c] = v;
-            }
+            for (int j = 0; j < kGrp; ++j) pdef = fma(r.re[j], 2.0 * g.rw[j], pdef);
+            pdef = wave_sum(pdef);
+            if (lane == 0) S.gred[wid][2 * kFitMaxBath] = pdef;
             __syncthreads();
             if (tid < kFitCols) {
                 const int c = tid, k = c < kFitMaxBath ? c : c - kFitMaxBath;
-                if (c == 2 * kFitMaxBath || k < nb) {
+                if (c == 2 * kFitMaxBath || (k < nb && S.xt[nb + o * nb + k] != 0.0)) {
                     double v = 0.0;
                     for (int w = 0; w < kFitThreads / 64; ++w) v = v + S.gred[w][c];
-                    const bool zero_v = c != 2 * kFitMaxBath && S.xt[nb + o * nb + k] == 0.0;
-                    if (c < kFitMaxBath) {  // d/de_k: accumulated over the orbital passes
-                        if (o == 0)
-                            S.gacc[k] = zero_v ? 0.0 : v;
-                        else if (!zero_v)
-                            S.gacc[k] = S.gacc[k] + v;
-                    } else if (c < 2 * kFitMaxBath) {
-                        S.gacc[nb + o * nb + k] = zero_v ? 0.0 : v;
-                    } else {
-                        S.gacc[q.nx - q.norb + o] = v;
-                    }
+                    const int comp = c < kFitMaxBath ? k : (c < 2 * kFitMaxBath ? nb + o * nb + k : q.nx - q.norb + o);
+                    S.gacc[comp] = S.gacc[comp] + v;
                 }
             }
             __syncthreads();
         }
-        for (int c = tid; c < q.nx; c += kFitThreads) out[c] = S.gacc[c] / scale1 / scale2;
     }
+    for (int c = tid; c < q.nx; c += kFitThreads) out[c] = S.gacc[c] / scale1 / scale2;
     __syncthreads();
 }
 
