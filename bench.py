@@ -7,6 +7,18 @@ line with the algorithmic throughput 16*D bytes per H v (read x once, write y on
 the host cores over MPI ranks as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--config ns14_roofline]
+
+With --dmft it instead runs the DMFT self-consistency loop of main.f90 on the 15-site, three-orbital
+sector (7, 8) (D = 41,409,225), 4096 frequencies at beta = 200, the tight-binding lattice step over
+Nq = 10^4 k-points and --nfit bath fits per iteration in one launch, and prints one JSON line with
+the wall time of each stage of each iteration:
+
+    python bench.py --dmft [--dmft-iters 8] [--nfit 256] [--fit-order tree|reference]
+
+With --solve CONFIG it times one impurity solve (ed_solve, then ed_calc_green_ftn) of a named
+workload, all of its sectors in one multi-problem run with --batched:
+
+    python bench.py --solve ns14_roofline | ns10_batched [--batched]
 """
 import argparse
 import json
@@ -107,6 +119,68 @@ def hbm_traffic(path):
     return None
 
 
+def dmft_bench(args):
+    """the DMFT loop (main.f90:59-127) at the 15-site fit workload; progress on stderr"""
+    import numpy as np
+    import moed
+    from moed import dmft
+    from moed.model import config, omega_grid
+
+    cfg = config("ns15_fit")
+    flags = moed.MOED_F_FIT_TREE if args.fit_order == "tree" else 0
+    nq = 10000
+    om = omega_grid(cfg.nw, cfg.beta)
+    hk = moed.band_tb(cfg.norb, nq)
+    t0 = time.perf_counter()
+    with moed.EDSolver.from_config(cfg, flags=flags) as s:
+        nu, nd = cfg.sectors[0]
+
+        def report(r):
+            print(f"bench.py --dmft: iteration {r['iteration']}: {r['t_total']:.2f} s (solve {r['t_solve']:.2f}, "
+                  f"green {r['t_green']:.2f}, lattice {r['t_delta']:.3f}, fit {r['t_fit']:.2f}), "
+                  f"chi2 {r['fret']:.4e}, scf diff {r['scf_diff']:.3e}", file=sys.stderr, flush=True)
+
+        recs, _ = dmft.dmft_loop(s, hk, cfg.mu, om, args.dmft_iters, nfit=args.nfit, scf_tol=0.0, on_iter=report)
+        D = s.dim(nu, nd)
+    wall = time.perf_counter() - t0
+    keys = ("t_solve", "t_green", "t_delta", "t_fit", "t_total")
+    out = {
+        "metric": "DMFT loop wall s, 15-site 3-orbital sector, bath fits per iteration in one launch",
+        "value": round(sum(r["t_total"] for r in recs), 3), "unit": "s", "n_gpus": 1,
+        "steps": len(recs), "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"DMFT loop, {cfg.nsite}-site sector N_up={nu} N_dn={nd}, D={D}, t_ij on",
+                   "norb": cfg.norb, "nbath": cfg.nbath, "U": cfg.U, "J": cfg.J, "nw": cfg.nw, "beta": cfg.beta,
+                   "nq": nq, "mu": cfg.mu, "nfit": args.nfit, "fit_order": args.fit_order,
+                   "nstep": cfg.nstep},
+        "per_stage_s": {k: round(sum(r[k] for r in recs), 3) for k in keys},
+        "iterations": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in recs],
+        "wall_s_with_setup": round(wall, 3),
+    }
+    print(json.dumps(out))
+
+
+def solve_bench(args):
+    """one ed_solve + ed_calc_green_ftn of a named workload"""
+    import moed
+    from moed.model import config, omega_grid
+
+    cfg = config(args.solve)
+    flags = moed.MOED_F_BATCHED if args.batched else 0
+    om = omega_grid(cfg.nw, cfg.beta)
+    with moed.EDSolver.from_config(cfg, flags=flags) as s:
+        t0 = time.perf_counter()
+        kept = s.ed_solve()
+        t1 = time.perf_counter()
+        s.ed_calc_green_ftn(om)
+        t2 = time.perf_counter()
+    out = {"metric": "impurity solve wall s (ed_solve + ed_calc_green_ftn)", "value": round(t2 - t0, 4), "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{cfg.name}: {cfg.nsite} sites, {len(cfg.sectors)} sector(s), nev {cfg.nev}, "
+                                  f"nstep {cfg.nstep}, {cfg.nw} frequencies", "batched": bool(args.batched)},
+           "t_solve_s": round(t1 - t0, 4), "t_green_s": round(t2 - t1, 4), "E0": kept[0][0], "states": len(kept)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,7 +194,17 @@ def main():
     ap.add_argument("--copy-check", type=int, default=0,
                     help="after the timed region, N plain device copies of the same vector (a kernel of known "
                          "traffic, 16*D bytes, that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--dmft", action="store_true", help="time the DMFT loop at the 15-site fit workload instead")
+    ap.add_argument("--dmft-iters", type=int, default=8)
+    ap.add_argument("--nfit", type=int, default=256)
+    ap.add_argument("--fit-order", choices=("tree", "reference"), default="tree")
+    ap.add_argument("--solve", default=None, help="time one impurity solve of this workload instead")
+    ap.add_argument("--batched", action="store_true", help="with --solve: all sectors in one multi-problem run")
     args = ap.parse_args()
+    if args.dmft:
+        return dmft_bench(args)
+    if args.solve:
+        return solve_bench(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the H v benchmark runs one sector on one GPU; a run on more than one GPU is not "
               "supported", file=sys.stderr)

@@ -129,6 +129,19 @@ int moed_green_record(moed_handle* h, int32_t irec, int32_t* iev, int32_t* iorb,
  * coefficient set; g[nw] complex. */
 int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
                   int32_t nw, const double* omega, double* g);
+/* Real-axis spectral function (dos.f90).
+ * moed_dos_diag: dos_diag (dos.f90:65-106) for one record: aff[w] = -Im(G+ + G-)/pi with
+ *   z+ = cmplx(ev + w, small), z- = cmplx(w - ev, small) and the green_diag recurrences over
+ *   ap, bp (0..nsp) and an, bn (0..nsn).
+ * moed_dos: dos (dos.f90:1-63) over the records of the last moed_green, in apbpanbn.dat order:
+ *   Aff(Norb, nw) column-major.  weighted = 0 reproduces the reference, where each record overwrites
+ *   its orbital's row with no probability or spin weight (dos.f90:102), so the last record per
+ *   orbital remains; weighted = 1 sums p * factor * A over the records (factor 1 for N_up = N_dn,
+ *   else 1/2, as ed_calc_green_ftn weights G).  The reference's grid is w_i = (i - nw/2) * 0.01
+ *   (main.f90:187-193) and small = DMFT.small (0.05, ed_config.f90:116). */
+int moed_dos_diag(moed_handle* h, int32_t nw, const double* omega_real, double small, double ev, int32_t nsp,
+                  int32_t nsn, const double* ap, const double* bp, const double* an, const double* bn, double* aff);
+int moed_dos(moed_handle* h, int32_t nw, const double* omega_real, double small, int32_t weighted, double* Aff);
 /* adjoint of moed_green_cf: given gbar = dL/dRe g + i dL/dIm g (complex[nw]), returns
  * ga[j] = dL/da_j, gb[j] = dL/db_j for j = 0..nsc.  No reference counterpart: it
  * differentiates G(iw) with respect to the Lanczos coefficients. */

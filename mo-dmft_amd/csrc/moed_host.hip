@@ -1273,6 +1273,74 @@ int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const dou
     return MOED_OK;
 }
 
+static int dos_record(moed_handle* h, int nw, const double* d_om, double small, double ev, int nsp, int nsn,
+                      const double* ap, const double* bp, const double* an, const double* bn, double weight,
+                      int accumulate, double* d_out, int stride) {
+    hipStream_t st = h->stream;
+    DevBuf ab;
+    const size_t np = nsp + 1, nn = nsn + 1;
+    HIPCHK(ab.ensure(2 * (np + nn) * sizeof(double)));
+    double* d = ab.as<double>();
+    HIPCHK(hipMemcpyAsync(d, ap, np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + np, bp, np * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + 2 * np, an, nn * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d + 2 * np + nn, bn, nn * sizeof(double), hipMemcpyHostToDevice, st));
+    const int compat = (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1;
+    DosArgs a{d, d + np, d + 2 * np, d + 2 * np + nn, nsp, nsn, ev, small, weight, d_om, nw, compat, accumulate,
+              d_out, stride};
+    HIPCHK(launch_dos(a, st));
+    HIPCHK(hipStreamSynchronize(st));  // ab is released on return
+    return MOED_OK;
+}
+
+int moed_dos_diag(moed_handle* h, int32_t nw, const double* omega_real, double small, double ev, int32_t nsp,
+                  int32_t nsn, const double* ap, const double* bp, const double* an, const double* bn, double* aff) {
+    if (!h) return MOED_EARG;
+    if (nw < 1 || nsp < 0 || nsn < 0 || !omega_real || !ap || !bp || !an || !bn || !aff)
+        return fail(h, MOED_EARG, "dos_diag: bad arguments");
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    DevBuf om, out;
+    HIPCHK(om.ensure(nw * sizeof(double)));
+    HIPCHK(out.ensure(nw * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(om.p, omega_real, nw * sizeof(double), hipMemcpyHostToDevice, st));
+    int rc = dos_record(h, nw, om.as<double>(), small, ev, nsp, nsn, ap, bp, an, bn, 1.0, 0, out.as<double>(), 1);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(aff, out.p, nw * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+int moed_dos(moed_handle* h, int32_t nw, const double* omega_real, double small, int32_t weighted, double* Aff) {
+    if (!h) return MOED_EARG;
+    if (nw < 1 || !omega_real || !Aff) return fail(h, MOED_EARG, "dos: bad arguments");
+    if (h->records.empty()) return fail(h, MOED_ESTATE, "dos: no Green's-function records (call moed_green first)");
+    (void)hipSetDevice(h->device);
+    hipStream_t st = h->stream;
+    const int norb = h->norb;
+    DevBuf om, out;
+    HIPCHK(om.ensure(nw * sizeof(double)));
+    HIPCHK(out.ensure((size_t)norb * nw * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(om.p, omega_real, nw * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_fill(0.0, (size_t)norb * nw, out.as<double>(), st));
+    // Aff(iorb, :) column-major (norb, nw): element (o, w) at o + norb w
+    for (size_t i = 0; i < h->records.size(); ++i) {
+        const GreenRecord& r = h->records[i];
+        if (!weighted) {  // every record overwrites its orbital: only the last one per orbital remains
+            bool later = false;
+            for (size_t j = i + 1; j < h->records.size() && !later; ++j) later = h->records[j].iorb == r.iorb;
+            if (later) continue;
+        }
+        const double weight = r.p * (r.even ? 1.0 : 0.5);
+        int rc = dos_record(h, nw, om.as<double>(), small, r.E, r.nsp, r.nsn, r.ap.data(), r.bp.data(), r.an.data(),
+                            r.bn.data(), weight, weighted ? 1 : 0, out.as<double>() + (r.iorb - 1), norb);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(Aff, out.p, (size_t)norb * nw * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
 int moed_green_cf_adjoint(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
                           int32_t nw, const double* omega, const double* gbar, double* ga, double* gb) {
     if (!h || !a || !b || !omega || !gbar || !ga || !gb || nsc < 0 || nw < 1) return MOED_EARG;

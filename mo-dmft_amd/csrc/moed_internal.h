@@ -199,6 +199,22 @@ struct CfArgs {
     int32_t stride;
 };
 hipError_t launch_cf(const CfArgs& a, hipStream_t s);
+
+// real-axis spectral function of one record (dos_diag, dos.f90:65-106):
+// out[i * stride] = -Im(G+ + G-)(w_i) / pi, or out += weight * that when accumulate
+struct DosArgs {
+    const double* ap;  // ap[0..nsp], bp[0..nsp], an[0..nsn], bn[0..nsn] (device)
+    const double* bp;
+    const double* an;
+    const double* bn;
+    int32_t nsp, nsn;
+    double ev, small, weight;
+    const double* omega;  // real frequencies
+    int32_t nw, compat, accumulate;
+    double* out;
+    int32_t stride;
+};
+hipError_t launch_dos(const DosArgs& a, hipStream_t s);
 hipError_t launch_cf_adjoint(const CfArgs& a, const double* gbar, double* scratch, double* ga, double* gb,
                              hipStream_t s);
 

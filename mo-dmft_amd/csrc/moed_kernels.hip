@@ -216,6 +216,49 @@ hipError_t launch_cf(const CfArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// dos_diag (dos.f90:87-103): z+ = cmplx(ev + w, small), z- = cmplx(w - ev, small) (REAL*4 parts in
+// compat mode), the two recurrences of green_diag, then -aimag(G+ + G-)/pi
+__global__ void dos_kernel(DosArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double sab[];  // ap, bp, an, bn
+    const int np = a.nsp + 1, nn = a.nsn + 1;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        sab[k] = a.ap[k];
+        sab[np + k] = a.bp[k];
+    }
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        sab[2 * np + k] = a.an[k];
+        sab[2 * np + nn + k] = a.bn[k];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nw) return;
+    const double* AP = sab;
+    const double* BP = sab + np;
+    const double* AN = sab + 2 * np;
+    const double* BN = sab + 2 * np + nn;
+    const double w = a.omega[i];
+    const double zi = a.compat ? round_f32(a.small) : a.small;
+    double zr = w + a.ev;
+    zr = a.compat ? round_f32(zr) : zr;
+    cxd gp = cx_rdiv(BP[a.nsp], {zr - AP[a.nsp], zi});
+    for (int j = a.nsp - 1; j >= 0; --j) gp = cx_rdiv(BP[j], {(zr - AP[j]) - gp.re, zi - gp.im});
+    zr = w - a.ev;
+    zr = a.compat ? round_f32(zr) : zr;
+    cxd gn = cx_rdiv(BN[a.nsn], {zr + AN[a.nsn], zi});
+    for (int j = a.nsn - 1; j >= 0; --j) gn = cx_rdiv(BN[j], {(zr + AN[j]) - gn.re, zi - gn.im});
+    const double pi = 0x1.921fb54442d18p+1;  // 4 atan(1) (ed_config.f90:7)
+    const double v = -(gp.im + gn.im) / pi;
+    double* o = a.out + (size_t)i * a.stride;
+    *o = a.accumulate ? *o + a.weight * v : v;
+}
+
+hipError_t launch_dos(const DosArgs& a, hipStream_t s) {
+    const size_t lds = (size_t)2 * (a.nsp + 1 + a.nsn + 1) * sizeof(double);
+    hipLaunchKernelGGL(dos_kernel, dim3((a.nw + 255) / 256), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
 // Adjoint of the continued fraction.  With D_j = z -+ a_j - t_{j+1}, t_j = b_j / D_j, g = t_0:
 // dg/dt_j = P_j = prod_{k<j} t_k / D_k, dg/db_j = P_j / D_j, dg/da_j = +-P_j t_j / D_j.
 // For real theta: dL/dtheta = Re(conj(gbar) dg/dtheta), gbar = dL/dRe g + i dL/dIm g.

@@ -130,6 +130,50 @@ module moed_capi
             real(c_double), value :: beta
             real(c_double), intent(out) :: omega(*)
         end function
+        integer(c_int) function moed_set_fit_launch(h, iterations) bind(C, name="moed_set_fit_launch")
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: iterations
+        end function
+        integer(c_int) function moed_fit_nfev(h, nfit, nfev) bind(C, name="moed_fit_nfev")
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nfit
+            integer(c_int32_t), intent(out) :: nfev(*)
+        end function
+        integer(c_int) function moed_band_tb(norb, nq, hk) bind(C, name="moed_band_tb")
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: norb, nq
+            real(c_double), intent(out) :: hk(*)
+        end function
+        integer(c_int) function moed_delta_new(h, nw, nq, rmu, hk, gr, dev, gksum, nocc) bind(C, name="moed_delta_new")
+            import :: c_ptr, c_int, c_int32_t, c_double, c_double_complex
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nw, nq
+            real(c_double), value :: rmu
+            real(c_double), intent(in) :: hk(*)
+            complex(c_double_complex), intent(in) :: gr(*)
+            complex(c_double_complex), intent(inout) :: dev(*)
+            complex(c_double_complex), intent(out) :: gksum(*)
+            real(c_double), intent(out) :: nocc(*)
+        end function
+        integer(c_int) function moed_dos_diag(h, nw, omega_real, small, ev, nsp, nsn, ap, bp, an, bn, aff) &
+                bind(C, name="moed_dos_diag")
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nw, nsp, nsn
+            real(c_double), intent(in) :: omega_real(*), ap(*), bp(*), an(*), bn(*)
+            real(c_double), value :: small, ev
+            real(c_double), intent(out) :: aff(*)
+        end function
+        integer(c_int) function moed_dos(h, nw, omega_real, small, weighted, aff) bind(C, name="moed_dos")
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: nw, weighted
+            real(c_double), intent(in) :: omega_real(*)
+            real(c_double), value :: small
+            real(c_double), intent(out) :: aff(*)
+        end function
     end interface
 end module moed_capi
 
@@ -141,7 +185,7 @@ module moed_ed
     private
     public :: basis_t, eigval_t, moed_ed_init, moed_ed_finalize, moed_ed_set_bath, generate_basis, multiply_H, &
               apply_c, lanczos_iteration, ed_solve, ed_calc_green_ftn, calc_dev, minimization, &
-              eigval, nev_calc, Gr, omega, nwloc, D_ev, moed_h
+              ed_delta_new_gpu, dos, eigval, nev_calc, Gr, omega, nwloc, D_ev, moed_h, dos_small
 
     ! ed_basis.F90:10-30 (one process owns the whole sector: nloc = ntot)
     type basis_t
@@ -162,6 +206,8 @@ module moed_ed
     double precision, allocatable, save :: omega(:)
     double complex, allocatable, save :: Gr(:,:), D_ev(:,:)
     integer, save :: m_norb = 0, m_nsite = 0, m_nev = 0, m_nstep = 0
+    ! DMFT.small of the reference (ed_config.f90:116), the broadening dos uses
+    double precision, save :: dos_small = 0.05d0
 
     ! storage the handle points into must outlive moed_create
     double precision, allocatable, target, save :: s_ek(:), s_vk(:)
@@ -330,5 +376,28 @@ contains
         if (st(1) /= 0) call check(int(st(1), c_int), 'minimization: frprmn maximum iterations exceeded')
         dab = fret(1)
     end subroutine minimization
+
+    ! ed_delta_new + n_from_gksum (ed_green.F90:187-253) on the device, for the caller's Gr, D_ev
+    ! (updated in place) and Gksum; hkd(o + Norb*(q-1)) = dble(Hk(o,o,q)) (moed_band_tb gives the
+    ! reference's tight-binding band)
+    subroutine ed_delta_new_gpu(Grl, Devl, Gksuml, rMu, Nq, hkd, nocc)
+        double complex, intent(in) :: Grl(:,:)
+        double complex, intent(inout) :: Devl(:,:)
+        double complex, intent(out) :: Gksuml(:,:)
+        double precision, intent(in) :: rMu, hkd(:)
+        integer, intent(in) :: Nq
+        double precision, intent(out) :: nocc(:)
+        call check(moed_delta_new(moed_h, int(size(Grl, 2), c_int32_t), int(Nq, c_int32_t), rMu, hkd, Grl, Devl, &
+                                  Gksuml, nocc), 'ed_delta_new')
+    end subroutine ed_delta_new_gpu
+
+    ! dos (dos.f90:1-63) with the reference's argument list, from the records of the last
+    ! ed_calc_green_ftn; each record overwrites its orbital's row as in the reference
+    subroutine dos(omega_real, Aff)
+        double precision, intent(in) :: omega_real(:)
+        double precision, intent(out) :: Aff(:,:)
+        call check(moed_dos(moed_h, int(size(omega_real), c_int32_t), omega_real, dos_small, 0_c_int32_t, Aff), &
+                   'dos')
+    end subroutine dos
 
 end module moed_ed

@@ -59,7 +59,7 @@ EXPORTS = [
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
     "moed_fill_start_device", "moed_set_hv_shape", "moed_stream", "moed_band_tb", "moed_delta_new",
-    "moed_set_fit_launch", "moed_fit_nfev",
+    "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos",
 ]
 
 _lib = None
@@ -101,6 +101,9 @@ def lib():
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
         L.moed_set_fit_launch.argtypes = [ct.c_void_p, ct.c_int32]
+        L.moed_dos_diag.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_double, ct.c_int32, ct.c_int32,
+                                    _dp, _dp, _dp, _dp, _dp]
+        L.moed_dos.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_int32, _dp]
         L.moed_fit_nfev.argtypes = [ct.c_void_p, ct.c_int32, _ip]
         L.moed_band_tb.argtypes = [ct.c_int32, ct.c_int32, _dp]
         L.moed_delta_new.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_double, _dp, _dp, _dp, _dp, _dp]
@@ -333,6 +336,24 @@ class EDSolver:
         self._check(lib().moed_minimize(self.h, nfit, self.nx, _d(x), len(omega), _d(omega), _d(D), ftol, _d(fret),
                                         it.ctypes.data_as(_ip), st.ctypes.data_as(_ip)))
         return x, fret, it, st
+
+    # ------------------------------------------------------------ real-axis spectral function
+
+    def dos_diag(self, omega_real, small, ev, ap, bp, an, bn):
+        """dos_diag (dos.f90:65-106) for one record: A(w) = -Im(G+ + G-)/pi."""
+        om = _f64(omega_real)
+        ap, bp, an, bn = _f64(ap), _f64(bp), _f64(an), _f64(bn)
+        out = np.empty(len(om))
+        self._check(lib().moed_dos_diag(self.h, len(om), _d(om), float(small), float(ev), len(ap) - 1, len(an) - 1,
+                                        _d(ap), _d(bp), _d(an), _d(bn), _d(out)))
+        return out
+
+    def dos(self, omega_real, small=0.05, weighted=False):
+        """dos (dos.f90:1-63) over the records of the last ed_calc_green_ftn: Aff (norb, nw)."""
+        om = _f64(omega_real)
+        out = np.empty(self.norb * len(om))
+        self._check(lib().moed_dos(self.h, len(om), _d(om), float(small), int(bool(weighted)), _d(out)))
+        return out.reshape(self.norb, len(om), order="F")
 
     # ------------------------------------------------------------ lattice step
     def delta_new(self, Gr, Dev, hk, rmu):

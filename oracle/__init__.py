@@ -70,6 +70,8 @@ def lib():
         L.or_sort.argtypes = [ct.c_int, _dp, _ip]
         L.or_omega_grid.argtypes = [ct.c_int, ct.c_double, _dp]
         L.or_green_cf.argtypes = [ct.c_int, ct.c_double, ct.c_int, _dp, _dp, ct.c_int, _dp, ct.c_int, _dp]
+        L.or_dos_diag.argtypes = [ct.c_int, _dp, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp,
+                                  ct.c_int, _dp]
         L.or_green_diag.argtypes = [ct.POINTER(_Model), ct.POINTER(_Basis), _dp, ct.c_double, ct.c_double,
                                     ct.c_double, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, ct.c_int, _dp,
                                     _ip, _ip, _dp, _dp, _dp, _dp, ct.c_int]
@@ -227,6 +229,32 @@ def green_diag(model, basis, eigvec, E, pev, factor, iorb, ispin, nstep, omega, 
     if rc:
         raise ValueError("green_diag: bad sector")
     return G[0::2] + 1j * G[1::2], (nsp.value, ap, bp), (nsn.value, an, bn)
+
+
+def dos_diag(omega_real, small, ev, ap, bp, an, bn, compat=True):
+    """dos_diag (dos.f90:65-106) for one record: A(w) = -Im(G+ + G-)/pi at z = (ev +- w, small)."""
+    om = _f64(omega_real)
+    ap, bp, an, bn = _f64(ap), _f64(bp), _f64(an), _f64(bn)
+    out = np.empty(len(om))
+    lib().or_dos_diag(len(om), _d(om), float(small), float(ev), len(ap) - 1, len(an) - 1, _d(ap), _d(bp), _d(an),
+                      _d(bn), int(compat), _d(out))
+    return out
+
+
+def dos(records, norb, omega_real, small, weighted=False, compat=True):
+    """dos (dos.f90:1-63) over Green's-function records in file order: each record overwrites its
+    orbital's row (the reference applies no probability or spin weight, dos.f90:102).  weighted=True
+    instead sums p * factor * A over the records (factor 1 when even, else 1/2, as ed_green.F90:87-94)."""
+    aff = np.zeros((norb, len(omega_real)))
+    for r in records:
+        nsp, nsn = r["nsp"], r["nsn"]
+        a = dos_diag(omega_real, small, r["E"], r["ap"][:nsp + 1], r["bp"][:nsp + 1], r["an"][:nsn + 1],
+                     r["bn"][:nsn + 1], compat)
+        if weighted:
+            aff[r["iorb"] - 1] += (r["p"] * (1.0 if r["even"] else 0.5)) * a
+        else:
+            aff[r["iorb"] - 1] = a
+    return aff
 
 
 def calc_dev(model, ef, omega, compat=True):

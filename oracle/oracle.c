@@ -613,6 +613,24 @@ void or_green_cf(int sign, double E, int n, const double* a, const double* b, in
     }
 }
 
+/* dos_diag, dos.f90:65-106: A(w) = -Im(G+(w) + G-(w)) / pi on the real axis for one record, with
+ * z+ = cmplx(ev + w, small) and z- = cmplx(w - ev, small) (default-kind cmplx: REAL*4 parts when
+ * compat); the recurrences are green_diag's.  pi = 4 atan(1) (ed_config.f90:7). */
+void or_dos_diag(int nw, const double* omega_real, double small, double ev, int nsp, int nsn, const double* ap,
+                 const double* bp, const double* an, const double* bn, int compat, double* aff) {
+    const double pi = 4.0 * atan(1.0);
+    for (int i = 0; i < nw; i++) {
+        cx z = cx_mk(cround(ev + omega_real[i], compat), cround(small, compat));
+        cx gp = cx_rdiv(bp[nsp], cx_subr(z, ap[nsp]));
+        for (int j = nsp - 1; j >= 0; j--) gp = cx_rdiv(bp[j], cx_sub(cx_subr(z, ap[j]), gp));
+        z = cx_mk(cround(omega_real[i] - ev, compat), cround(small, compat));
+        cx gn = cx_rdiv(bn[nsn], cx_addr(z, an[nsn]));
+        for (int j = nsn - 1; j >= 0; j--) gn = cx_rdiv(bn[j], cx_sub(cx_addr(z, an[j]), gn));
+        const cx grx = cx_add(gp, gn);
+        aff[i] = -grx.im / pi;
+    }
+}
+
 /* green_diag, ed_green.F90:136-185 */
 int or_green_diag(const or_model* m, const or_basis* b, const double* eigvec, double E, double pev, double factor,
                   int iorb, int ispin, int nstep, int nw, const double* omega, int compat, double* Gr, int* nsp,

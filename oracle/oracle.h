@@ -117,4 +117,6 @@ void or_n_from_gksum(int norb, int nw, double beta, const double* gksum, double*
 #ifdef __cplusplus
 }
 #endif
+void or_dos_diag(int nw, const double* omega_real, double small, double ev, int nsp, int nsn, const double* ap,
+                 const double* bp, const double* an, const double* bn, int compat, double* aff);
 #endif

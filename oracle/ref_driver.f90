@@ -21,6 +21,8 @@
 !   9 ed_delta_new (ed_green.F90:187) after ed_set_band_structure (ed_hamiltonian.F90:29)
 !                 i1 = nw, i2 = Nq; payload rMu, Gr(2*norb*nw), D_ev(2*norb*nw)
 !                 -> Hk(i,i,q) real parts (norb*Nq), Gksum(2*norb*nw), D_ev(2*norb*nw)
+!  10 dos_diag     (dos.f90:65)            i1 = nw, i2 = nstep_calcp, i3 = nstep_calcn; payload small, ev,
+!                 int32 even, omega_real(nw), ap, bp (0:nsp), an, bn (0:nsn) -> Aff(1, 1:nw) for iorb = 1
 program ref_driver
     use parallel_params
     use ed_config
@@ -36,6 +38,9 @@ program ref_driver
     double precision, allocatable :: x(:), y(:), a(:), b(:), om(:), dv(:), g(:)
     double complex, allocatable :: dev(:,:)
     integer, allocatable :: indx(:), rows(:)
+    integer :: ieven
+    logical :: even
+    double precision, allocatable :: ap(:), bp(:), an(:), bn(:), aff(:,:)
     type(basis_t) :: bas, bout
     double precision, external :: func
 
@@ -161,6 +166,21 @@ program ref_driver
         write(11) ((dble(Hk(i, i, j)), i = 1, norb), j = 1, Nq)
         write(11) Gksum
         write(11) D_ev
+    case (10)
+        nw = i1
+        read(10) uj
+        small = uj(1)
+        read(10) ieven
+        even = ieven .ne. 0
+        allocate(om(nw), ap(0:i2), bp(0:i2), an(0:i3), bn(0:i3), aff(norb, nw))
+        read(10) om
+        read(10) ap
+        read(10) bp
+        read(10) an
+        read(10) bn
+        aff = 0.0d0
+        call dos_diag(1, aff, om, 1, uj(2), 1.0d0, even, i2, i3, ap, bp, an, bn)
+        write(11) aff(1, :)
     end select
     if (node .eq. 0) close(11)
     close(10)

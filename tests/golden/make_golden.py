@@ -147,6 +147,29 @@ def gen_ref_delta():
     print("  reference lattice-step fixture written")
 
 
+def gen_ref_dos():
+    """dos_diag (dos.f90:65-106) on the reference's real-axis grid w_i = (i - nw/2) * 0.01 (main.f90:187-193):
+    three records with different recurrence lengths, signs of E and broadening"""
+    nw = 400
+    om = (np.arange(1, nw + 1) - nw // 2) * 0.01
+    rng = np.random.default_rng(2718)
+    cases = []
+    for nsp, nsn, ev, small in ((0, 0, -1.5, 0.05), (12, 19, -7.25, 0.05), (40, 33, 0.375, 0.1)):
+        ap, bp = rng.uniform(-3, 3, nsp + 1), rng.uniform(0.05, 1.2, nsp + 1)
+        an, bn = rng.uniform(-3, 3, nsn + 1), rng.uniform(0.05, 1.2, nsn + 1)
+        m = oracle.Model(1, 1, 0.0, 0.0, np.zeros(2), np.zeros((1, 1)))
+        raw = oracle.run_ref(10, m, i1=nw, i2=nsp, i3=nsn,
+                             payload=[np.array([small, ev]), np.array([1], np.int32), om, ap, bp, an, bn])
+        cases.append(dict(nsp=nsp, nsn=nsn, ev=ev, small=small, ap=ap, bp=bp, an=an, bn=bn,
+                          aff=np.frombuffer(raw, "<f8").copy()))
+    flat = {}
+    for i, c in enumerate(cases):
+        for k, v in c.items():
+            flat[f"{k}_{i}"] = v
+    np.savez_compressed(os.path.join(HERE, "ref_dos.npz"), ncase=len(cases), omega=om, **flat)
+    print("  reference spectral-function fixture written")
+
+
 def gen_dense(cfgs):
     kat = dict(source="mathematica/MO-DMFT-ED.nb:1498-1505 (parameters), :1555-1592 (matrix)",
                norb=2, nbath=1, U=400.0, J=200.0, ek=[-100.0, -100.0, 0.5], vk=[20.0, 10.0], sector=[2, 1],
@@ -181,14 +204,19 @@ def main():
     ap.add_argument("--eigsh", action="store_true")
     ap.add_argument("--eigsh-big", action="store_true", help="also the 14-site sector (slow)")
     ap.add_argument("--delta-only", action="store_true", help="only the lattice-step fixture")
+    ap.add_argument("--dos-only", action="store_true", help="only the spectral-function fixture")
     args = ap.parse_args()
     cfgs = load_configs()
     if args.delta_only:
         gen_ref_delta()
         return
+    if args.dos_only:
+        gen_ref_dos()
+        return
     if oracle.ref_available():
         gen_ref(cfgs)
         gen_ref_delta()
+        gen_ref_dos()
     else:
         print("reference build not available; skipping reference fixtures")
     e0_path = os.path.join(HERE, "e0.json")

@@ -56,3 +56,10 @@ def test_fortran_interface_compiles():
         r = subprocess.run([fc, "-c", src, "-module-dir", td, "-o", os.path.join(td, "m.o")], capture_output=True,
                            text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_fortran_bindings_name_declared_symbols():
+    """every bind(C, name=...) in the Fortran module is a function include/moed.h declares"""
+    src = open(os.path.join(ROOT, "mo-dmft_amd", "fortran", "moed_iface.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(moed_[a-z0-9_]+)"\)', src))
+    assert bound and bound <= set(declared_functions()), sorted(bound - set(declared_functions()))

@@ -321,3 +321,53 @@ def test_ev_to_x_roundtrip():
     x = oracle.ev_to_x(ek, vk, ef, norb, nbath)
     ef2, ek2, vk2 = oracle.x_to_ev(x, norb, nbath)
     assert np.array_equal(ef2, ef) and np.array_equal(ek2, ek) and np.array_equal(vk2, vk)
+
+
+# ------------------------------------------------------------------ real-axis spectral function
+
+def test_dos_diag_matches_reference_build():
+    """dos_diag (dos.f90:65-106) against the reference build's outputs, bit for bit"""
+    z = np.load(os.path.join(GOLD, "ref_dos.npz"))
+    for i in range(int(z["ncase"])):
+        a = oracle.dos_diag(z["omega"], float(z[f"small_{i}"]), float(z[f"ev_{i}"]), z[f"ap_{i}"], z[f"bp_{i}"],
+                            z[f"an_{i}"], z[f"bn_{i}"])
+        assert np.array_equal(a, z[f"aff_{i}"]), i
+
+
+def test_dos_diag_single_level_closed_form():
+    """one-step fractions: A(w) = -Im(b+/(w + E + i eta - a+) + b-/(w - E + i eta + a-))/pi, evaluated
+    with exact double z (no REAL*4 rounding) against numpy's complex arithmetic"""
+    om = np.linspace(-4, 4, 801)
+    E, eta, ap, bp, an, bn = -0.75, 0.05, 0.3, 0.6, -0.2, 0.4
+    a = oracle.dos_diag(om, eta, E, [ap], [bp], [an], [bn], compat=False)
+    g = bp / (om + E + 1j * eta - ap) + bn / (om - E + 1j * eta + an)
+    assert np.max(np.abs(a - (-g.imag / np.pi))) <= 1e-14 * np.abs(g.imag / np.pi).max()
+    # a Lorentzian of weight bp + bn: its integral over a wide grid
+    wide = np.linspace(-400, 400, 800001)
+    aw = oracle.dos_diag(wide, eta, E, [ap], [bp], [an], [bn], compat=False)
+    assert abs(np.trapezoid(aw, wide) - (bp + bn)) <= 1e-3
+
+
+def test_dos_records_overwrite_and_weighted():
+    """dos.f90:17-59: every record overwrites its orbital's row; the weighted variant sums
+    p * factor * A (factor 1/2 when N_up != N_dn)"""
+    om = (np.arange(1, 101) - 50) * 0.01
+    rng = np.random.default_rng(6)
+    recs = []
+    for iev in (1, 2):
+        for iorb in (1, 2):
+            ap, bp, an, bn = (rng.uniform(0.1, 1, 6) for _ in range(4))
+            recs.append(dict(iev=iev, iorb=iorb, E=-1.0 - iev, p=0.5 * iev, even=iev == 1, nsp=3, nsn=5,
+                             ap=ap, bp=bp, an=an, bn=bn))
+    last = oracle.dos(recs, 2, om, 0.05)
+    for o in (0, 1):
+        r = recs[2 + o]
+        want = oracle.dos_diag(om, 0.05, r["E"], r["ap"][:4], r["bp"][:4], r["an"], r["bn"])
+        assert np.array_equal(last[o], want)
+    w = oracle.dos(recs, 2, om, 0.05, weighted=True)
+    for o in (0, 1):
+        acc = np.zeros(len(om))
+        for r in recs[o::2]:
+            acc = acc + (r["p"] * (1.0 if r["even"] else 0.5)) * oracle.dos_diag(
+                om, 0.05, r["E"], r["ap"][:4], r["bp"][:4], r["an"], r["bn"])
+        assert np.array_equal(w[o], acc)
