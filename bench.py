@@ -153,7 +153,7 @@ def dmft_bench(args):
                    "nq": nq, "mu": cfg.mu, "nfit": args.nfit, "fit_order": args.fit_order,
                    "nstep": cfg.nstep},
         "per_stage_s": {k: round(sum(r[k] for r in recs), 3) for k in keys},
-        "iterations": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in recs],
+        "iterations": [{k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()} for r in recs],
         "wall_s_with_setup": round(wall, 3),
     }
     print(json.dumps(out))
