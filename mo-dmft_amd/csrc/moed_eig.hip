@@ -209,10 +209,10 @@ hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b,
 }
 
 // part[i * ni + item] = <V_i, w> over the item for the VG basis vectors of group blockIdx.y
-// (i < k1), and <w, w> in slot k1 when with_self
+// (i0 <= i < k1)
 constexpr int kEigVG = 8;
 __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, const double* V, const double* w,
-                                                                   int k1, int with_self, int pass) {
+                                                                   int i0, int k1, int pass) {
     __shared__ double red[kEigVG][kEigThreads / 64];
     const int item = blockIdx.x;
     const int p = e.item_prob[item];
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
         const int r = threadIdx.x + j * kEigThreads;
         wv[j] = r < len ? w[base + r] : 0.0;
     }
-    const int g0 = blockIdx.y * kEigVG;
+    const int g0 = i0 + blockIdx.y * kEigVG;
     double acc[kEigVG];
 #pragma unroll
     for (int q = 0; q < kEigVG; ++q) {
@@ -239,9 +239,6 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
                 const int r = threadIdx.x + j * kEigThreads;
                 if (r < len) s = fma(u[r], wv[j], s);
             }
-        } else if (i == k1 && with_self) {
-#pragma unroll
-            for (int j = 0; j < R; ++j) s = fma(wv[j], wv[j], s);
         }
         acc[q] = wave_sum(s);
     }
@@ -253,7 +250,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
     __syncthreads();
     if (threadIdx.x < kEigVG) {
         const int i = g0 + threadIdx.x;
-        if (i < k1 + with_self) {
+        if (i < k1) {
             double s = 0.0;
             for (int wv2 = 0; wv2 < kEigThreads / 64; ++wv2) s += red[threadIdx.x][wv2];
             e.part[(size_t)i * e.ni + item] = s;
@@ -261,13 +258,13 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
     }
 }
 
-// h_i = sum of the item partials (one wave per vector i); c_i = h_i / nrm_i^2; the wave of vector
-// k adds h_k / nrm_k to the step's alpha, the wave of slot k1 stores |w|^2
-__global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, int k1, int with_self, int pass) {
+// h_i = sum of the item partials (one wave per vector i, i0 <= i < k1); c_i = h_i / nrm_i^2; the
+// wave of vector k adds h_k / nrm_k to the step's alpha
+__global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, int i0, int k1, int pass) {
     const int p = blockIdx.x;
     if (!eig_active(e, p, pass)) return;
-    const int i = blockIdx.y * (kEigThreads / 64) + (threadIdx.x >> 6);
-    if (i >= k1 + with_self) return;
+    const int i = i0 + blockIdx.y * (kEigThreads / 64) + (threadIdx.x >> 6);
+    if (i >= k1) return;
     const double sum = wave_item_sum(e.part, (size_t)i * e.ni + e.pitem0[p], e.pnitem[p]);
     if ((threadIdx.x & 63) != 0) return;
     const double* nrm = e.nrm + (size_t)p * e.ldh;
@@ -275,12 +272,11 @@ __global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, in
     e.h[(size_t)p * e.ldh + i] = sum;
     if (i < k1) e.c[(size_t)p * e.ldh + i] = sum / (nrm[i] * nrm[i]);
     if (i == k1 - 1) aux[0] += sum / nrm[i];
-    if (i == k1) aux[1] = sum;
 }
 
-// dst = src - sum_{i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item
+// dst = src - sum_{i0 <= i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item
 __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, const double* V, const double* src,
-                                                                  double* dst, int k1, int pass) {
+                                                                  double* dst, int i0, int k1, int pass) {
     extern __shared__ double sc[];
     __shared__ double red[kEigThreads / 64];
     const int item = blockIdx.x;
@@ -299,7 +295,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
     }
     // (a full item: no bounds tests, eight basis vectors' loads in flight per step)
     if (len == kEigChunk) {
-        int i = 0;
+        int i = i0;
         for (; i + 8 <= k1; i += 8) {
             double u[8][R];
 #pragma unroll
@@ -315,7 +311,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
 #pragma unroll
             for (int j = 0; j < R; ++j) acc[j] = fma(-sc[i], V[(size_t)i * e.dtot + base + threadIdx.x + j * kEigThreads], acc[j]);
     } else {
-        for (int i = 0; i < k1; ++i) {
+        for (int i = i0; i < k1; ++i) {
             const double ci = sc[i];
             const double* __restrict__ u = V + (size_t)i * e.dtot + base;
 #pragma unroll
@@ -338,7 +334,9 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
     if (threadIdx.x == 0) e.part[item] = ss;
 }
 
-// |w'|^2 from the item partials; DGKS test (pass 1); the step's alpha, beta and the stop test
+// |w'|^2 from the item partials.  Pass 0 (the local step against V_{k-1}, V_k) records it as the
+// reference norm; pass 1 (the whole basis) applies the DGKS test against it; after pass 1 or 2 the
+// step's alpha, beta and the stop test
 __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
     const int p = blockIdx.x;
     if (!eig_active(e, p, pass)) return;
@@ -346,6 +344,10 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
     if (threadIdx.x != 0) return;
     double* aux = e.aux + (size_t)p * 4;
     aux[2] = wp2;
+    if (pass == 0) {
+        aux[1] = wp2;
+        return;
+    }
     if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
         e.pass2[p] = 1;
         aux[1] = wp2;
@@ -368,18 +370,21 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
     }
 }
 
+// One Gram-Schmidt pass of step k.  Pass 0: w (= H V_k) against V_{k-1}, V_k only, the three-term
+// part, written to slot k + 1; pass 1: slot k + 1 against the whole basis V_0..V_k, in place;
+// pass 2 (only where pass 1's DGKS test asks for it): pass 1 again.
 hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
                            double tiny, hipStream_t s) {
     const int k1 = k + 1;
-    const int with_self = pass == 1 ? 1 : 0;
-    const double* src = pass == 1 ? w : V + (size_t)k1 * e.dtot;
+    const int i0 = pass == 0 ? (k > 0 ? k - 1 : 0) : 0;
+    const double* src = pass == 0 ? w : V + (size_t)k1 * e.dtot;
     double* dst = const_cast<double*>(V) + (size_t)k1 * e.dtot;
-    const int ng = (k1 + with_self + kEigVG - 1) / kEigVG;
-    hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, k1, with_self, pass);
-    hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P, (k1 + with_self + 3) / 4), dim3(kEigThreads), 0, s, e, k1,
-                       with_self, pass);
+    const int nv = k1 - i0;
+    const int ng = (nv + kEigVG - 1) / kEigVG;
+    hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, i0, k1, pass);
+    hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P, (nv + 3) / 4), dim3(kEigThreads), 0, s, e, i0, k1, pass);
     hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * sizeof(double), s, e, V, src,
-                       dst, k1, pass);
+                       dst, i0, k1, pass);
     hipLaunchKernelGGL(eig_finish_kernel, dim3(e.P), dim3(64), 0, s, e, k, pass, eta2, tiny);
     return hipGetLastError();
 }

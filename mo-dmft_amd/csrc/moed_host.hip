@@ -671,8 +671,9 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
             int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
             if (rc) return rc;
         }
-        HIPCHK(launch_eig_pass(e, V, W, k, 1, 0.5, 1e-14, st));
-        HIPCHK(launch_eig_pass(e, V, W, k, 2, 0.5, 1e-14, st));
+        HIPCHK(launch_eig_pass(e, V, W, k, 0, 0.5, 1e-14, st));  // three-term part
+        HIPCHK(launch_eig_pass(e, V, W, k, 1, 0.5, 1e-14, st));  // whole basis
+        HIPCHK(launch_eig_pass(e, V, W, k, 2, 0.5, 1e-14, st));  // again where DGKS asks
         const int m = k + 1;
         if (m < 20 || m % 5 == 0 || m == M) {
             HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));

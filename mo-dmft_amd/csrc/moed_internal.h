@@ -138,9 +138,10 @@ struct EigArgs {
 // part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
                              hipStream_t s);
-// one reorthogonalisation pass at step k (basis V_0..V_k, packed with stride dtot):
-//   pass 1: h_i = <V_i, w>, |w|^2; V_{k+1} = w - sum_i c_i V_i; DGKS test
-//   pass 2: (problems flagged) the same with w = V_{k+1}, in place; then the step's alpha, beta
+// one Gram-Schmidt pass at step k (basis V_0..V_k, packed with stride dtot), c_i = <V_i, src> / |V_i|^2:
+//   pass 0: V_{k+1} = w - sum_{i = k-1, k} c_i V_i (the three-term part); |V_{k+1}|^2 is the DGKS reference
+//   pass 1: V_{k+1} -= sum_{i <= k} c_i V_i in place; DGKS test against the reference
+//   pass 2: (problems flagged) pass 1 again; after pass 1 or 2 the step's alpha, beta and stop test
 hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
                            double tiny, hipStream_t s);
 // Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
