@@ -129,6 +129,24 @@ int moed_green_record(moed_handle* h, int32_t irec, int32_t* iev, int32_t* iorb,
  * coefficient set; g[nw] complex. */
 int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const double* a, const double* b,
                   int32_t nw, const double* omega, double* g);
+/* On-disk records, in the reference's Fortran unformatted sequential layout (4-byte record
+ * lengths around every record).
+ * moed_eigvec_filename: eigvec_filename (ed_io.F90:141-147), "eigvec-" I2.2 "-" I3.3 "-" I4.4; a
+ *   field that does not fit prints as asterisks, as in the reference (sectors >= 100 give "**").
+ * moed_export_eigvec / moed_import_eigvec: export_eigvec / import_eigvec (ed_io.F90:13-33,
+ *   105-139): records (nloc) and (eigvec(1:nloc)) in dir/<filename>; import with vec = NULL
+ *   returns nloc only.  Host only.
+ * moed_export_green: apbpanbn.dat as ed_calc_green_ftn writes it (ed_green.F90:75, 112-129): per
+ *   record (nstep_calcp, nstep_calcn), then (iev, iorb, E, p, even, ap, bp, an, bn), from the
+ *   records of the last moed_green.
+ * moed_import_green: reads such a file (written by the reference or by moed_export_green) into the
+ *   handle's records, for moed_dos and moed_green_record. */
+int moed_eigvec_filename(int32_t isector, int32_t iev, int32_t node, char* buf, int32_t len);
+int moed_export_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t nloc, const double* vec);
+int moed_import_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t* nloc, double* vec);
+int moed_export_green(moed_handle* h, const char* path);
+int moed_import_green(moed_handle* h, const char* path);
+
 /* Real-axis spectral function (dos.f90).
  * moed_dos_diag: dos_diag (dos.f90:65-106) for one record: aff[w] = -Im(G+ + G-)/pi with
  *   z+ = cmplx(ev + w, small), z- = cmplx(w - ev, small) and the green_diag recurrences over

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <string>
@@ -1339,6 +1340,174 @@ int moed_dos(moed_handle* h, int32_t nw, const double* omega_real, double small,
     }
     HIPCHK(hipMemcpyAsync(Aff, out.p, (size_t)norb * nw * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+// ---------------------------------------------------------------- on-disk records
+
+// Fortran unformatted sequential files: every record is a 4-byte little-endian length, the
+// payload, and the length again (records up to 2^31 - 1 bytes)
+extern "C++" {
+namespace {
+struct RecordWriter {
+    std::string buf;
+    template <class T>
+    void put(const T& v) {
+        buf.append(reinterpret_cast<const char*>(&v), sizeof(T));
+    }
+    void put(const double* v, size_t n) { buf.append(reinterpret_cast<const char*>(v), n * sizeof(double)); }
+    bool flush(FILE* f) {
+        if (buf.size() > 0x7fffffffull) return false;
+        const int32_t n = (int32_t)buf.size();
+        const bool ok = fwrite(&n, 4, 1, f) == 1 && (buf.empty() || fwrite(buf.data(), buf.size(), 1, f) == 1) &&
+                        fwrite(&n, 4, 1, f) == 1;
+        buf.clear();
+        return ok;
+    }
+};
+
+// the next record's payload, or false at the end of the file or on a malformed record
+bool read_record(FILE* f, std::string* out) {
+    int32_t n = 0, m = 0;
+    if (fread(&n, 4, 1, f) != 1 || n < 0) return false;
+    out->resize(n);
+    if (n > 0 && fread(&(*out)[0], n, 1, f) != 1) return false;
+    return fread(&m, 4, 1, f) == 1 && m == n;
+}
+
+// Fortran edit descriptor Iw.m for a non-negative field: at least m digits in w columns, w
+// asterisks when the value does not fit (as the reference's file names do for sector >= 100)
+std::string fortran_i(int64_t v, int w, int m) {
+    char b[32];
+    std::snprintf(b, sizeof b, "%0*lld", m, (long long)v);
+    std::string s(b);
+    if (v < 0 || (int)s.size() > w) return std::string(w, '*');
+    return std::string(w - s.size(), ' ') + s;
+}
+
+// eigvec_filename (ed_io.F90:141-147): "eigvec-" I2.2 "-" I3.3 "-" I4.4
+std::string eigvec_path(const char* dir, int isector, int iev, int node) {
+    std::string name = "eigvec-" + fortran_i(isector, 2, 2) + "-" + fortran_i(iev, 3, 3) + "-" + fortran_i(node, 4, 4);
+    if (dir && *dir) return std::string(dir) + "/" + name;
+    return name;
+}
+}  // namespace
+}  // extern "C++"
+
+int moed_eigvec_filename(int32_t isector, int32_t iev, int32_t node, char* buf, int32_t len) {
+    if (!buf || len <= 0) return MOED_EARG;
+    std::snprintf(buf, (size_t)len, "%s", eigvec_path(nullptr, isector, iev, node).c_str());
+    return MOED_OK;
+}
+
+int moed_export_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t nloc, const double* vec) {
+    if (nloc < 0 || (nloc > 0 && !vec) || nloc * 8 > 0x7fffffffll) return MOED_EARG;
+    FILE* f = std::fopen(eigvec_path(dir, isector, iev, node).c_str(), "wb");
+    if (!f) return MOED_EARG;
+    RecordWriter w;
+    w.put((int32_t)nloc);
+    bool ok = w.flush(f);
+    w.put(vec, (size_t)nloc);
+    ok = ok && w.flush(f);
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? MOED_OK : MOED_EARG;
+}
+
+int moed_import_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t* nloc, double* vec) {
+    if (!nloc) return MOED_EARG;
+    FILE* f = std::fopen(eigvec_path(dir, isector, iev, node).c_str(), "rb");
+    if (!f) return MOED_EARG;  // 'import_eigvec: File not found' (ed_io.F90:120-124)
+    std::string r1, r2;
+    const bool ok = read_record(f, &r1) && r1.size() == 4 && read_record(f, &r2);
+    std::fclose(f);
+    if (!ok) return MOED_EARG;
+    int32_t n;
+    std::memcpy(&n, r1.data(), 4);
+    if (n < 0 || r2.size() != (size_t)n * 8) return MOED_EARG;
+    *nloc = n;
+    if (vec) std::memcpy(vec, r2.data(), r2.size());
+    return MOED_OK;
+}
+
+int moed_export_green(moed_handle* h, const char* path) {
+    if (!h || !path) return MOED_EARG;
+    if (h->records.empty()) return fail(h, MOED_ESTATE, "export_green: no records (call moed_green first)");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(h, MOED_EARG, std::string("export_green: cannot open ") + path);
+    bool ok = true;
+    RecordWriter w;
+    for (const GreenRecord& r : h->records) {  // ed_green.F90:112-114
+        w.put((int32_t)r.nsp);
+        w.put((int32_t)r.nsn);
+        ok = ok && w.flush(f);
+        w.put((int32_t)r.iev);
+        w.put((int32_t)r.iorb);
+        w.put(r.E);
+        w.put(r.p);
+        w.put((int32_t)(r.even ? 1 : 0));  // default LOGICAL, 4 bytes
+        w.put(r.ap.data(), r.nsp + 1);
+        w.put(r.bp.data(), r.nsp + 1);
+        w.put(r.an.data(), r.nsn + 1);
+        w.put(r.bn.data(), r.nsn + 1);
+        ok = ok && w.flush(f);
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? MOED_OK : fail(h, MOED_EARG, "export_green: write failed");
+}
+
+int moed_import_green(moed_handle* h, const char* path) {
+    if (!h || !path) return MOED_EARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(h, MOED_EARG, std::string("import_green: cannot open ") + path);
+    std::vector<GreenRecord> recs;
+    std::string a, b;
+    bool ok = true;
+    int last_iev = 0, count_in_iev = 0;
+    while (read_record(f, &a)) {
+        int32_t hd[2];
+        if (a.size() != 8 || !read_record(f, &b)) {
+            ok = false;
+            break;
+        }
+        std::memcpy(hd, a.data(), 8);
+        GreenRecord r{};
+        r.nsp = hd[0];
+        r.nsn = hd[1];
+        const size_t need = 4 + 4 + 8 + 8 + 4 + 8 * (size_t)(2 * (r.nsp + 1) + 2 * (r.nsn + 1));
+        if (r.nsp < 0 || r.nsn < 0 || b.size() != need) {
+            ok = false;
+            break;
+        }
+        const char* q = b.data();
+        int32_t i4;
+        std::memcpy(&i4, q, 4), q += 4, r.iev = i4;
+        std::memcpy(&i4, q, 4), q += 4, r.iorb = i4;
+        std::memcpy(&r.E, q, 8), q += 8;
+        std::memcpy(&r.p, q, 8), q += 8;
+        std::memcpy(&i4, q, 4), q += 4, r.even = i4 != 0;
+        auto take = [&](std::vector<double>& v, int n) {
+            v.resize(n);
+            std::memcpy(v.data(), q, 8 * (size_t)n);
+            q += 8 * (size_t)n;
+        };
+        take(r.ap, r.nsp + 1);
+        take(r.bp, r.nsp + 1);
+        take(r.an, r.nsn + 1);
+        take(r.bn, r.nsn + 1);
+        if (r.iorb < 1 || r.iorb > h->norb) {
+            ok = false;
+            break;
+        }
+        // the file does not store the spin: per state, Norb spin-up records, then (N_up != N_dn)
+        // Norb spin-down records (ed_green.F90:107-131)
+        count_in_iev = r.iev == last_iev ? count_in_iev + 1 : 0;
+        last_iev = r.iev;
+        r.ispin = count_in_iev < h->norb ? 1 : 2;
+        recs.push_back(std::move(r));
+    }
+    std::fclose(f);
+    if (!ok) return fail(h, MOED_EARG, "import_green: malformed apbpanbn record");
+    h->records = std::move(recs);
     return MOED_OK;
 }
 

@@ -166,6 +166,23 @@ module moed_capi
             real(c_double), value :: small, ev
             real(c_double), intent(out) :: aff(*)
         end function
+        integer(c_int) function moed_export_green(h, path) bind(C, name="moed_export_green")
+            import :: c_ptr, c_int, c_char
+            type(c_ptr), value :: h
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function moed_import_green(h, path) bind(C, name="moed_import_green")
+            import :: c_ptr, c_int, c_char
+            type(c_ptr), value :: h
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function moed_export_eigvec(dir, isector, iev, node, nloc, vec) bind(C, name="moed_export_eigvec")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_char
+            character(kind=c_char), intent(in) :: dir(*)
+            integer(c_int32_t), value :: isector, iev, node
+            integer(c_int64_t), value :: nloc
+            real(c_double), intent(in) :: vec(*)
+        end function
         integer(c_int) function moed_dos(h, nw, omega_real, small, weighted, aff) bind(C, name="moed_dos")
             import :: c_ptr, c_int, c_int32_t, c_double
             type(c_ptr), value :: h

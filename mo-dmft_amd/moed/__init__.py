@@ -59,7 +59,8 @@ EXPORTS = [
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
     "moed_fill_start_device", "moed_set_hv_shape", "moed_stream", "moed_band_tb", "moed_delta_new",
-    "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos",
+    "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
+    "moed_export_eigvec", "moed_import_eigvec", "moed_export_green", "moed_import_green",
 ]
 
 _lib = None
@@ -104,6 +105,12 @@ def lib():
         L.moed_dos_diag.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_double, ct.c_int32, ct.c_int32,
                                     _dp, _dp, _dp, _dp, _dp]
         L.moed_dos.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_int32, _dp]
+        L.moed_eigvec_filename.argtypes = [ct.c_int32, ct.c_int32, ct.c_int32, ct.c_char_p, ct.c_int32]
+        L.moed_export_eigvec.argtypes = [ct.c_char_p, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int64, _dp]
+        L.moed_import_eigvec.argtypes = [ct.c_char_p, ct.c_int32, ct.c_int32, ct.c_int32, ct.POINTER(ct.c_int64),
+                                         _dp]
+        L.moed_export_green.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.moed_import_green.argtypes = [ct.c_void_p, ct.c_char_p]
         L.moed_fit_nfev.argtypes = [ct.c_void_p, ct.c_int32, _ip]
         L.moed_band_tb.argtypes = [ct.c_int32, ct.c_int32, _dp]
         L.moed_delta_new.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_double, _dp, _dp, _dp, _dp, _dp]
@@ -135,6 +142,36 @@ def omega_grid(nw, beta):
     if rc:
         raise MoedError(rc, "omega grid")
     return om
+
+
+def eigvec_filename(isector, iev, node=0):
+    """eigvec_filename (ed_io.F90:141-147)"""
+    buf = ct.create_string_buffer(64)
+    rc = lib().moed_eigvec_filename(isector, iev, node, buf, 64)
+    if rc:
+        raise MoedError(rc, "eigvec_filename")
+    return buf.value.decode()
+
+
+def export_eigvec(directory, isector, iev, vec, node=0):
+    """export_eigvec (ed_io.F90:13-33): the reference's unformatted eigenvector file"""
+    v = _f64(vec)
+    rc = lib().moed_export_eigvec(os.fsencode(directory), isector, iev, node, len(v), _d(v))
+    if rc:
+        raise MoedError(rc, "export_eigvec")
+
+
+def import_eigvec(directory, isector, iev, node=0):
+    """import_eigvec (ed_io.F90:105-139)"""
+    n = ct.c_int64()
+    rc = lib().moed_import_eigvec(os.fsencode(directory), isector, iev, node, ct.byref(n), None)
+    if rc:
+        raise MoedError(rc, "import_eigvec: missing or malformed file")
+    v = np.empty(n.value)
+    rc = lib().moed_import_eigvec(os.fsencode(directory), isector, iev, node, ct.byref(n), _d(v))
+    if rc:
+        raise MoedError(rc, "import_eigvec")
+    return v
 
 
 def band_tb(norb, nq):
@@ -336,6 +373,23 @@ class EDSolver:
         self._check(lib().moed_minimize(self.h, nfit, self.nx, _d(x), len(omega), _d(omega), _d(D), ftol, _d(fret),
                                         it.ctypes.data_as(_ip), st.ctypes.data_as(_ip)))
         return x, fret, it, st
+
+    # ------------------------------------------------------------ on-disk records
+
+    def export_green(self, path):
+        """apbpanbn.dat of the last ed_calc_green_ftn (ed_green.F90:112-129)"""
+        self._check(lib().moed_export_green(self.h, os.fsencode(path)))
+
+    def import_green(self, path):
+        """read an apbpanbn.dat file into the solver's Green's-function records"""
+        self._check(lib().moed_import_green(self.h, os.fsencode(path)))
+
+    def export_eigvecs(self, directory):
+        """the eigenvector files ed_solve writes (ed_solver.F90:58-61), every sector and level"""
+        for isec in range(1, len(self.sectors) + 1):
+            ev = self.sector_eigenvalues(isec)
+            for lev in range(1, int(np.sum(ev < 1e300)) + 1):  # levels with an eigenvector
+                export_eigvec(directory, isec, lev, self.eigvec(isec, lev))
 
     # ------------------------------------------------------------ real-axis spectral function
 

@@ -23,6 +23,11 @@
 !                 -> Hk(i,i,q) real parts (norb*Nq), Gksum(2*norb*nw), D_ev(2*norb*nw)
 !  10 dos_diag     (dos.f90:65)            i1 = nw, i2 = nstep_calcp, i3 = nstep_calcn; payload small, ev,
 !                 int32 even, omega_real(nw), ap, bp (0:nsp), an, bn (0:nsn) -> Aff(1, 1:nw) for iorb = 1
+!  11 export_eigvec (ed_io.F90:13)          i1 = isector, i2 = iev, i3 = nloc; payload eigvec(nloc)
+!                 -> file name (150 characters), size in bytes (int32), the file's bytes
+!  12 apbpanbn.dat records as ed_calc_green_ftn writes them (ed_green.F90:75, 112-114): i1 = number
+!                 of records; per record payload int32 nsp, nsn, iev, iorb, even; real64 val, prob,
+!                 ap, bp (0:nsp), an, bn (0:nsn) -> size in bytes (int32), the file's bytes
 program ref_driver
     use parallel_params
     use ed_config
@@ -31,6 +36,7 @@ program ref_driver
     use ed_hamiltonian
     use ed_lanczos
     use ed_green, only: nwloc, Gr, D_ev, Gksum, ed_delta_new
+    use ed_io, only: export_eigvec
     implicit none
     character(len=512) :: fin, fout
     integer :: hdr(8), task, ne_up, ne_dn, i1, i2, i3, i, j, nnz, nstep_calc, nx, nwl
@@ -38,8 +44,10 @@ program ref_driver
     double precision, allocatable :: x(:), y(:), a(:), b(:), om(:), dv(:), g(:)
     double complex, allocatable :: dev(:,:)
     integer, allocatable :: indx(:), rows(:)
-    integer :: ieven
+    integer :: ieven, nbytes, rhdr(5)
     logical :: even
+    character(len=150) :: fname
+    integer(kind=1), allocatable :: bytes(:)
     double precision, allocatable :: ap(:), bp(:), an(:), bn(:), aff(:,:)
     type(basis_t) :: bas, bout
     double precision, external :: func
@@ -181,8 +189,47 @@ program ref_driver
         aff = 0.0d0
         call dos_diag(1, aff, om, 1, uj(2), 1.0d0, even, i2, i3, ap, bp, an, bn)
         write(11) aff(1, :)
+    case (11)
+        allocate(x(i3))
+        read(10) x
+        call export_eigvec(i1, i2, 0, i3, 1, x)
+        write(fname, "(A7,I2.2,A1,I3.3,A1,I4.4)") "eigvec-", i1, "-", i2, "-", 0
+        call copy_file(trim(fname))
+    case (12)
+        open(89, file="apbpanbn.dat", form="unformatted", status="new")
+        do j = 1, i1
+            read(10) rhdr
+            even = rhdr(5) .ne. 0
+            allocate(a(2), ap(0:rhdr(1)), bp(0:rhdr(1)), an(0:rhdr(2)), bn(0:rhdr(2)))
+            read(10) a
+            read(10) ap
+            read(10) bp
+            read(10) an
+            read(10) bn
+            write(89) rhdr(1), rhdr(2)
+            write(89) rhdr(3), rhdr(4), a(1), a(2), even, (ap(i), i = 0, rhdr(1)), (bp(i), i = 0, rhdr(1)), &
+                      (an(i), i = 0, rhdr(2)), (bn(i), i = 0, rhdr(2))
+            deallocate(a, ap, bp, an, bn)
+        enddo
+        close(89)
+        fname = "apbpanbn.dat"
+        call copy_file("apbpanbn.dat")
     end select
     if (node .eq. 0) close(11)
     close(10)
     call MPI_Finalize(ierr)
+contains
+    ! the bytes of a file written by the reference's I/O, appended to the output
+    subroutine copy_file(name)
+        character(len=*), intent(in) :: name
+        inquire(file=name, size=nbytes)
+        allocate(bytes(nbytes))
+        open(12, file=name, access='stream', form='unformatted', status='old')
+        read(12) bytes
+        close(12)
+        write(11) fname
+        write(11) nbytes
+        write(11) bytes
+        deallocate(bytes)
+    end subroutine copy_file
 end program ref_driver

@@ -17,6 +17,7 @@ Fixtures written (all small):
                   H v (12 and 14 sites)
 """
 import argparse
+import struct
 import hashlib
 import json
 import os
@@ -170,6 +171,35 @@ def gen_ref_dos():
     print("  reference spectral-function fixture written")
 
 
+def gen_ref_files():
+    """the reference's on-disk records: an eigenvector file written by export_eigvec (ed_io.F90:13)
+    and apbpanbn.dat records written with ed_calc_green_ftn's statements (ed_green.F90:112-114), as
+    the file bytes the reference build produces"""
+    m = oracle.Model(1, 1, 0.0, 0.0, np.zeros(2), np.zeros((1, 1)))
+    rng = np.random.default_rng(99)
+    vec = rng.standard_normal(37)
+    raw = oracle.run_ref(11, m, i1=7, i2=3, i3=len(vec), payload=[vec])
+    n = struct.unpack("<i", raw[150:154])[0]
+    eig_name, eig_bytes = raw[:150].decode().strip(), np.frombuffer(raw[154:154 + n], np.uint8).copy()
+    raw = oracle.run_ref(11, m, i1=121, i2=12, i3=3, payload=[vec[:3]])
+    eig_name_121 = raw[:150].decode().strip()
+    recs = [(4, 6, 1, 1, 1, -3.25, 0.625), (4, 6, 1, 2, 1, -3.25, 0.625), (0, 2, 2, 1, 0, -1.5, 0.1875),
+            (3, 0, 2, 2, 0, -1.5, 0.1875), (5, 5, 2, 1, 0, -1.5, 0.1875), (2, 7, 2, 2, 0, -1.5, 0.1875)]
+    payload, flat = [], {}
+    for i, (nsp, nsn, iev, iorb, even, val, prob) in enumerate(recs):
+        ab = [rng.uniform(-2, 2, nsp + 1), rng.uniform(0.1, 1, nsp + 1), rng.uniform(-2, 2, nsn + 1),
+              rng.uniform(0.1, 1, nsn + 1)]
+        payload += [np.array([nsp, nsn, iev, iorb, even], np.int32), np.array([val, prob])] + ab
+        flat.update({f"rec{i}_hdr": np.array([nsp, nsn, iev, iorb, even], np.int32), f"rec{i}_vp": np.array([val, prob]),
+                     f"rec{i}_ap": ab[0], f"rec{i}_bp": ab[1], f"rec{i}_an": ab[2], f"rec{i}_bn": ab[3]})
+    raw = oracle.run_ref(12, m, i1=len(recs), payload=payload)
+    n = struct.unpack("<i", raw[150:154])[0]
+    np.savez_compressed(os.path.join(HERE, "ref_files.npz"), vec=vec, eig_name=eig_name, eig_bytes=eig_bytes,
+                        eig_name_121=eig_name_121, nrec=len(recs),
+                        ab_bytes=np.frombuffer(raw[154:154 + n], np.uint8).copy(), **flat)
+    print("  reference on-disk record fixture written")
+
+
 def gen_dense(cfgs):
     kat = dict(source="mathematica/MO-DMFT-ED.nb:1498-1505 (parameters), :1555-1592 (matrix)",
                norb=2, nbath=1, U=400.0, J=200.0, ek=[-100.0, -100.0, 0.5], vk=[20.0, 10.0], sector=[2, 1],
@@ -205,6 +235,7 @@ def main():
     ap.add_argument("--eigsh-big", action="store_true", help="also the 14-site sector (slow)")
     ap.add_argument("--delta-only", action="store_true", help="only the lattice-step fixture")
     ap.add_argument("--dos-only", action="store_true", help="only the spectral-function fixture")
+    ap.add_argument("--files-only", action="store_true", help="only the on-disk record fixture")
     args = ap.parse_args()
     cfgs = load_configs()
     if args.delta_only:
@@ -213,10 +244,14 @@ def main():
     if args.dos_only:
         gen_ref_dos()
         return
+    if args.files_only:
+        gen_ref_files()
+        return
     if oracle.ref_available():
         gen_ref(cfgs)
         gen_ref_delta()
         gen_ref_dos()
+        gen_ref_files()
     else:
         print("reference build not available; skipping reference fixtures")
     e0_path = os.path.join(HERE, "e0.json")

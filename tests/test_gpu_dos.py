@@ -1,5 +1,6 @@
 """GPU parity of the real-axis spectral function (dos.f90): dos_diag against the reference build's
-outputs, and dos over the Green's-function records of a solve against the oracle."""
+outputs, and dos over the Green's-function records of a solve against the oracle; the on-disk
+records (apbpanbn.dat, eigenvector files) against the reference build's bytes and in round trips."""
 import os
 
 import numpy as np
@@ -44,3 +45,61 @@ def test_dos_after_green_matches_oracle(sectors):
             a = s.dos(om, 0.05, weighted=weighted)
             ref = oracle.dos(recs, norb, om, 0.05, weighted=weighted)
             assert np.array_equal(a, ref), weighted
+
+
+def _model():
+    rng = np.random.default_rng(21)
+    norb, nbath = 2, 4
+    ek = np.concatenate([np.full(norb, -1.5), rng.uniform(-2, 2, nbath)])
+    vk = np.zeros((norb, nbath))
+    for b in range(nbath):
+        vk[b % norb, b] = rng.uniform(0.3, 0.9)
+    return norb, nbath, ek, vk
+
+
+def test_apbpanbn_import_and_export_match_the_reference_bytes(tmp_path):
+    """a file written with ed_calc_green_ftn's record statements (reference build) reads back as
+    its records, and moed_export_green writes the same bytes"""
+    z = np.load(os.path.join(GOLD, "ref_files.npz"))
+    path = tmp_path / "apbpanbn.dat"
+    path.write_bytes(z["ab_bytes"].tobytes())
+    norb, nbath, ek, vk = _model()
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, nstep=10) as s:
+        s.import_green(str(path))
+        recs = s.green_records()
+        assert len(recs) == int(z["nrec"])
+        for i, r in enumerate(recs):
+            nsp, nsn, iev, iorb, even = (int(v) for v in z[f"rec{i}_hdr"])
+            assert (r["nsp"], r["nsn"], r["iev"], r["iorb"], r["even"]) == (nsp, nsn, iev, iorb, bool(even))
+            assert (r["E"], r["p"]) == tuple(z[f"rec{i}_vp"])
+            assert r["ispin"] == (1 if even or i in (2, 3) else 2)
+            for k, n in (("ap", nsp), ("bp", nsp), ("an", nsn), ("bn", nsn)):
+                assert np.array_equal(r[k][:n + 1], z[f"rec{i}_{k}"])
+        out = tmp_path / "again.dat"
+        s.export_green(str(out))
+        assert out.read_bytes() == z["ab_bytes"].tobytes()
+
+
+def test_records_and_eigenvectors_round_trip_after_a_solve(tmp_path):
+    """export after ed_solve / ed_calc_green_ftn, import into a fresh solver: the same records and
+    the same spectral function; every eigenvector file reads back as the device's eigenvector"""
+    import moed
+    norb, nbath, ek, vk = _model()
+    om = (np.arange(1, 201) - 100) * 0.01
+    args = dict(beta=20.0, nstep=30, nev=2, sectors=[(3, 2), (2, 3), (3, 3)])
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, **args) as s:
+        s.ed_solve()
+        s.ed_calc_green_ftn(omega_grid(64, 20.0))
+        s.export_green(str(tmp_path / "apbpanbn.dat"))
+        s.export_eigvecs(str(tmp_path))
+        recs, aff = s.green_records(), s.dos(om)
+        vecs = {(i, l): s.eigvec(i, l) for i in (1, 2, 3) for l in (1, 2)}
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, **args) as t:
+        t.import_green(str(tmp_path / "apbpanbn.dat"))
+        recs2 = t.green_records()
+        assert np.array_equal(t.dos(om), aff)
+    assert len(recs2) == len(recs)
+    for a, b in zip(recs, recs2):
+        assert all(np.array_equal(a[k], b[k]) for k in a)
+    for (i, l), v in vecs.items():
+        assert np.array_equal(moed.import_eigvec(str(tmp_path), i, l), v)

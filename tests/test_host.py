@@ -1,6 +1,8 @@
-"""Host-side logic of the package (no GPU): the DMFT loop's parameter packing and start points,
-and the band structure, against the oracle's restatements of the reference."""
+"""Host-side logic of the package (no GPU): the DMFT loop's parameter packing and start points and
+the band structure against the oracle's restatements of the reference; the eigenvector files
+against the reference build's bytes."""
 import numpy as np
+import pytest
 
 import oracle
 from moed import dmft
@@ -31,3 +33,44 @@ def test_start_points_keep_zero_couplings_and_the_current_bath():
 def test_band_structure_host_entry_point():
     import moed
     assert np.array_equal(moed.band_tb(3, 400), oracle.band_tb(3, 400))
+
+
+# ------------------------------------------------------------------ on-disk records (host only)
+
+def _files():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_files.npz"))
+
+
+def test_eigvec_file_is_byte_identical_to_the_reference(tmp_path):
+    """export_eigvec (ed_io.F90:13-33): the reference build's file, byte for byte"""
+    import moed
+    z = _files()
+    moed.export_eigvec(str(tmp_path), 7, 3, z["vec"])
+    name = moed.eigvec_filename(7, 3)
+    assert name == str(z["eig_name"])
+    assert (tmp_path / name).read_bytes() == z["eig_bytes"].tobytes()
+
+
+def test_eigvec_filename_overflows_like_fortran():
+    """I2.2 cannot hold sector 121: the reference prints asterisks (ed_io.F90:145)"""
+    import moed
+    assert moed.eigvec_filename(121, 12) == str(_files()["eig_name_121"])
+    assert moed.eigvec_filename(1, 1, 0) == "eigvec-01-001-0000"
+
+
+def test_import_eigvec_reads_the_reference_file(tmp_path):
+    import moed
+    z = _files()
+    (tmp_path / str(z["eig_name"])).write_bytes(z["eig_bytes"].tobytes())
+    assert np.array_equal(moed.import_eigvec(str(tmp_path), 7, 3), z["vec"])
+
+
+def test_import_eigvec_rejects_missing_and_malformed_files(tmp_path):
+    import moed
+    z = _files()
+    with pytest.raises(Exception):
+        moed.import_eigvec(str(tmp_path), 7, 3)
+    (tmp_path / str(z["eig_name"])).write_bytes(z["eig_bytes"].tobytes()[:-5])
+    with pytest.raises(Exception):
+        moed.import_eigvec(str(tmp_path), 7, 3)
