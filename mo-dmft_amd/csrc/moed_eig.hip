@@ -6,10 +6,12 @@
 //                        down-move rows are read directly.  Term order per element is that of
 //                        the two-pass product (diagonal, up slots, down moves, spin flip /
 //                        pair exchange), so both give bit-identical y.
-//   eig_* kernels        full-reorthogonalisation Lanczos (classical Gram-Schmidt with a DGKS
-//                        second pass) over packed vectors: per-item partial inner products,
-//                        per-problem reductions in a fixed order, projection, the step's
-//                        alpha / beta and the stop test, all on the device.
+//   eig_* kernels        Lanczos with partial reorthogonalisation over packed vectors: the
+//                        three-term projection every step, the whole basis (classical
+//                        Gram-Schmidt, DGKS second pass) where the estimated loss of
+//                        orthogonality asks for it; per-item partial inner products,
+//                        per-problem reductions in a fixed order, the step's alpha / beta and
+//                        the stop test, all on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -166,6 +168,7 @@ hipError_t launch_hv_batch(const HvBatchArgs& a, int nitems, int max_nU, int nor
 
 __device__ __forceinline__ bool eig_active(const EigArgs& e, int p, int pass) {
     if (e.done[p]) return false;
+    if (pass == 1) return e.full[p] != 0;
     return pass != 2 || e.pass2[p] != 0;
 }
 
@@ -334,26 +337,15 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
     if (threadIdx.x == 0) e.part[item] = ss;
 }
 
-// |w'|^2 from the item partials.  Pass 0 (the local step against V_{k-1}, V_k) records it as the
-// reference norm; pass 1 (the whole basis) applies the DGKS test against it; after pass 1 or 2 the
-// step's alpha, beta and the stop test
-__global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
-    const int p = blockIdx.x;
-    if (!eig_active(e, p, pass)) return;
-    const double wp2 = wave_item_sum(e.part, e.pitem0[p], e.pnitem[p]);
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// the step's alpha (accumulated over its passes) and beta = |V_{k+1}|, the exhaustion / cap test
+__device__ void eig_close_step(const EigArgs& e, int p, int k, double wp2, double tiny) {
     double* aux = e.aux + (size_t)p * 4;
-    aux[2] = wp2;
-    if (pass == 0) {
-        aux[1] = wp2;
-        return;
-    }
-    if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
-        e.pass2[p] = 1;
-        aux[1] = wp2;
-        return;
-    }
-    e.pass2[p] = 0;
     double* alpha = e.alpha + (size_t)p * e.ldh;
     double* beta = e.beta + (size_t)p * e.ldh;
     alpha[k] = aux[0];
@@ -370,9 +362,80 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
     }
 }
 
+// |w'|^2 from the item partials.
+// Pass 0 (the three-term part): partial reorthogonalisation (H. D. Simon, "The Lanczos algorithm
+// with partial reorthogonalization", Math. Comp. 42 (1984) 115-142).  With q_i = V_i / |V_i|, the
+// overlaps omega_{k+1,i} ~ <q_{k+1}, q_i> follow from the recurrence's alpha, beta:
+//   beta_k omega_{k+1,i} = beta_i omega_{k,i+1} + (alpha_i - alpha_k) omega_{k,i}
+//                          + beta_{i-1} omega_{k,i-1} - beta_{k-1} omega_{k-1,i} (+- eps |T|),
+// omega_{k+1,k} = eps |T| / beta_k, omega_{k+1,k+1} = 1.  When max_i<k |omega_{k+1,i}| exceeds
+// sqrt(eps), V_{k+1} is projected against the whole basis (pass 1, then pass 2 where the DGKS test
+// asks), and so is V_{k+2} at the next step; the estimates of a vector projected against the whole
+// basis drop to eps.  Otherwise the step closes here, and the basis stays semi-orthogonal
+// (|<q_i, q_j>| <= sqrt(eps)), which keeps the Ritz values exact to rounding.
+// Pass 1 applies the DGKS test against pass 0's norm; pass 1 or 2 closes the step.
+__global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pass, double eta2, double tiny) {
+    const int p = blockIdx.x;
+    if (!eig_active(e, p, pass)) return;
+    const int lane = threadIdx.x;
+    const double wp2 = __shfl(wave_item_sum(e.part, e.pitem0[p], e.pnitem[p]), 0, 64);
+    double* aux = e.aux + (size_t)p * 4;
+    const size_t ld = e.ldh;
+    double* om = e.om + (size_t)p * 3 * ld;
+    double* wn = om + (size_t)((k + 1) % 3) * ld;  // omega_{k+1}
+    const double eps = 0x1p-52;
+    if (pass == 0) {
+        const double* alpha = e.alpha + (size_t)p * ld;
+        const double* beta = e.beta + (size_t)p * ld;
+        const double* wc = om + (size_t)(k % 3) * ld;        // omega_k
+        const double* wp = om + (size_t)((k + 2) % 3) * ld;  // omega_{k-1}
+        const double ak = aux[0], bk = sqrt(wp2);
+        double tn = 0.0;  // |T| estimate: max over rows of |alpha| + neighbouring betas
+        for (int i = lane; i < k; i += 64) tn = fmax(tn, fabs(alpha[i]) + beta[i] + (i > 0 ? beta[i - 1] : 0.0));
+        tn = fmax(wave_max(tn), fabs(ak) + bk + (k > 0 ? beta[k - 1] : 0.0));
+        const double rt = eps * tn;
+        double mx = 0.0;
+        if (bk > 0.0) {
+            for (int i = lane; i < k; i += 64) {
+                double t = beta[i] * wc[i + 1] + (alpha[i] - ak) * wc[i] - beta[k - 1] * wp[i];
+                if (i > 0) t += beta[i - 1] * wc[i - 1];
+                t = (t + copysign(rt, t)) / bk;
+                wn[i] = t;
+                mx = fmax(mx, fabs(t));
+            }
+        }
+        mx = wave_max(mx);
+        if (lane != 0) return;
+        if (k == 0) om[0] = 1.0;  // omega_{0,0}
+        wn[k] = bk > 0.0 ? rt / bk : 0.0;
+        wn[k + 1] = 1.0;
+        aux[1] = wp2;
+        const bool need = bk > 0.0 && (mx > sqrt(eps) || e.force[p]);
+        if (need) {
+            e.full[p] = 1;
+            e.force[p] = e.force[p] ? 0 : 1;
+        } else {
+            e.full[p] = 0;
+            eig_close_step(e, p, k, wp2, tiny);
+        }
+        return;
+    }
+    if (lane != 0) return;
+    aux[2] = wp2;
+    if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
+        e.pass2[p] = 1;
+        aux[1] = wp2;
+        return;
+    }
+    e.pass2[p] = 0;
+    for (int i = 0; i < k; ++i) wn[i] = eps;  // V_{k+1} is now orthogonal to the basis to rounding
+    eig_close_step(e, p, k, wp2, tiny);
+}
+
 // One Gram-Schmidt pass of step k.  Pass 0: w (= H V_k) against V_{k-1}, V_k only, the three-term
-// part, written to slot k + 1; pass 1: slot k + 1 against the whole basis V_0..V_k, in place;
-// pass 2 (only where pass 1's DGKS test asks for it): pass 1 again.
+// part, written to slot k + 1; pass 1 (only where pass 0's overlap estimates ask for it): slot
+// k + 1 against the whole basis V_0..V_k, in place; pass 2 (only where pass 1's DGKS test asks
+// for it): pass 1 again.
 hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
                            double tiny, hipStream_t s) {
     const int k1 = k + 1;

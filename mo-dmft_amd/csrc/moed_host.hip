@@ -520,11 +520,12 @@ int eig_dense(moed_handle* h, const Sector& s, int nev, SectorEig& E) {
     return MOED_OK;
 }
 
-// Lanczos with full reorthogonalisation (classical Gram-Schmidt with a DGKS second pass) for
-// several sectors at once; one sector is the per-sector form of the same code.  Every problem p
-// runs the same iteration as alone: start vector u_0 (counter hash, seed of its sector), then per
-// step w = H u_k / |u_k|, u_{k+1} = w - sum_i <u_i, w>/|u_i|^2 u_i (second pass when |u_{k+1}|^2 <
-// |w|^2 / 2), alpha_k, beta_k = |u_{k+1}|.  A problem stops when the Krylov space is exhausted
+// Lanczos with partial reorthogonalisation for several sectors at once; one sector is the
+// per-sector form of the same code.  Every problem p runs the same iteration as alone: start
+// vector u_0 (counter hash, seed of its sector), then per step w = H u_k / |u_k|, u_{k+1} = w minus
+// its components along u_k, u_{k-1}; where the estimated loss of orthogonality exceeds sqrt(eps)
+// (moed_eig.hip, eig_finish_kernel), u_{k+1} -= sum_i <u_i, u_{k+1}>/|u_i|^2 u_i over the whole basis
+// (second pass when |u_{k+1}|^2 drops below half); alpha_k, beta_k = |u_{k+1}|.  A problem stops when the Krylov space is exhausted
 // (beta_k <= 1e-14 |T|), at its basis cap, or when the lowest nev Ritz pairs have relative
 // residual |beta_m z_{m,i}| <= tol |theta_i| (tested on the host after steps m < 20 and every
 // fifth step after, with one synchronisation for all problems).  Ritz vectors V S on the device.
@@ -580,21 +581,23 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     const size_t npart = std::max<size_t>((size_t)ldh * ni, (size_t)nev * ni);
     HIPCHK(g[EG_V].ensure((size_t)(M + 1) * dtot * sizeof(double)));
     HIPCHK(g[EG_W].ensure((size_t)dtot * sizeof(double)));
-    const size_t n32 = 2 * (size_t)ni + 7 * (size_t)P + 2 * (size_t)nr;
+    const size_t n32 = 2 * (size_t)ni + 9 * (size_t)P + 2 * (size_t)nr;
     HIPCHK(g[EG_I32].ensure(n32 * sizeof(int32_t)));
     HIPCHK(g[EG_I64].ensure(((size_t)ni + 2 * P) * sizeof(int64_t)));
-    const size_t ndbl = 5 * (size_t)P * ldh + 4 * (size_t)P + npart + (size_t)P * nev * ldh + (size_t)P * nev;
+    const size_t ndbl = 8 * (size_t)P * ldh + 4 * (size_t)P + npart + (size_t)P * nev * ldh + (size_t)P * nev;
     HIPCHK(g[EG_DBL].ensure(ndbl * sizeof(double)));
     HIPCHK(g[EG_DESC].ensure((size_t)P * sizeof(BatchProbDev)));
     HIPCHK(g[EG_OUT].ensure((size_t)P * sizeof(double*)));
     int32_t* i32 = g[EG_I32].as<int32_t>();
     int32_t *d_item_prob = i32, *d_item_len = d_item_prob + ni, *d_pitem0 = d_item_len + ni, *d_pnitem = d_pitem0 + P,
             *d_mmax = d_pnitem + P, *d_done = d_mmax + P, *d_pass2 = d_done + P, *d_mfinal = d_pass2 + P,
-            *d_nf = d_mfinal + P, *d_row_prob = d_nf + P, *d_row_id = d_row_prob + nr;
+            *d_nf = d_mfinal + P, *d_row_prob = d_nf + P, *d_row_id = d_row_prob + nr, *d_full = d_row_id + nr,
+            *d_force = d_full + P;
     int64_t *d_item_start = g[EG_I64].as<int64_t>(), *d_poff = d_item_start + ni, *d_pdim = d_poff + P;
     double* dbl = g[EG_DBL].as<double>();
     double *d_alpha = dbl, *d_beta = d_alpha + (size_t)P * ldh, *d_nrm = d_beta + (size_t)P * ldh,
-           *d_h = d_nrm + (size_t)P * ldh, *d_c = d_h + (size_t)P * ldh, *d_aux = d_c + (size_t)P * ldh,
+           *d_h = d_nrm + (size_t)P * ldh, *d_c = d_h + (size_t)P * ldh, *d_om = d_c + (size_t)P * ldh,
+           *d_aux = d_om + 3 * (size_t)P * ldh,
            *d_part = d_aux + 4 * (size_t)P, *d_S = d_part + npart, *d_n2 = d_S + (size_t)P * nev * ldh;
     std::vector<int32_t> zeroP(P, 0);
     HIPCHK(hipMemcpyAsync(d_item_prob, item_prob.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -605,6 +608,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     HIPCHK(hipMemcpyAsync(d_done, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pass2, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_mfinal, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_full, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_force, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     if (nr > 0) {
         HIPCHK(hipMemcpyAsync(d_row_prob, row_prob.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_row_id, row_id.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -613,7 +618,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     HIPCHK(hipMemcpyAsync(d_poff, off.data(), P * sizeof(int64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pdim, dim.data(), P * sizeof(int64_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(g[EG_DESC].p, desc.data(), P * sizeof(BatchProbDev), hipMemcpyHostToDevice, st));
-    HIPCHK(launch_fill(0.0, (int64_t)(5 * (size_t)P * ldh + 4 * (size_t)P), dbl, st));
+    HIPCHK(launch_fill(0.0, (int64_t)(8 * (size_t)P * ldh + 4 * (size_t)P), dbl, st));
 
     EigArgs e{};
     e.P = P;
@@ -638,6 +643,9 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     e.c = d_c;
     e.aux = d_aux;
     e.part = d_part;
+    e.om = d_om;
+    e.full = d_full;
+    e.force = d_force;
 
     double* V = g[EG_V].as<double>();
     double* W = g[EG_W].as<double>();

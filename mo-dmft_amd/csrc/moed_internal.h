@@ -134,13 +134,18 @@ struct EigArgs {
     double* c;                 // [P][ldh] projection coefficients
     double* aux;               // [P][4]: alpha accumulator, |w|^2, |w'|^2
     double* part;              // [(M + 2) * ni] per-item partial sums
+    double* om;                // [P][3][ldh] estimated overlaps omega_{k-1}, omega_k, omega_{k+1} (rotating)
+    int32_t* full;             // [P] 1: the current step projects against the whole basis
+    int32_t* force;            // [P] 1: the next step does too (the step after a triggered one)
 };
 // part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
                              hipStream_t s);
 // one Gram-Schmidt pass at step k (basis V_0..V_k, packed with stride dtot), c_i = <V_i, src> / |V_i|^2:
-//   pass 0: V_{k+1} = w - sum_{i = k-1, k} c_i V_i (the three-term part); |V_{k+1}|^2 is the DGKS reference
-//   pass 1: V_{k+1} -= sum_{i <= k} c_i V_i in place; DGKS test against the reference
+//   pass 0: V_{k+1} = w - sum_{i = k-1, k} c_i V_i (the three-term part); |V_{k+1}|^2 is the DGKS
+//           reference; the overlap estimates decide whether the step needs pass 1 (partial
+//           reorthogonalisation) — if not, the step's alpha, beta and stop test
+//   pass 1: (problems flagged) V_{k+1} -= sum_{i <= k} c_i V_i in place; DGKS test against the reference
 //   pass 2: (problems flagged) pass 1 again; after pass 1 or 2 the step's alpha, beta and stop test
 hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
                            double tiny, hipStream_t s);
