@@ -38,9 +38,7 @@ extern "C" {
 /* flags */
 #define MOED_F_EXACT_CMPLX 1u   /* evaluate G(iw), Delta(iw) at exact double E, w_n instead of the
                                    reference's default-kind cmplx() (REAL*4-rounded) values */
-#define MOED_F_GF_REORTH 2u     /* full reorthogonalisation in the Green's-function Lanczos runs
-                                   (the reference has none, ed_lanczos.F90:37-68) */
-#define MOED_F_NO_TUNING 4u     /* ignore the launch-shape tuning table */
+/* (bits 2u and 4u are reserved) */
 #define MOED_F_BATCHED 8u       /* moed_solve: all sectors in one multi-problem Lanczos run (one launch
                                    per operation for every sector) instead of sector by sector */
 #define MOED_F_FIT_TREE 16u     /* bath fit: sum chi^2 and its gradient over frequencies in a fixed tree
@@ -140,7 +138,8 @@ int moed_green_cf(moed_handle* h, int32_t sign, double E, int32_t nsc, const dou
  *   record (nstep_calcp, nstep_calcn), then (iev, iorb, E, p, even, ap, bp, an, bn), from the
  *   records of the last moed_green.
  * moed_import_green: reads such a file (written by the reference or by moed_export_green) into the
- *   handle's records, for moed_dos and moed_green_record. */
+ *   handle's records, for moed_dos and moed_green_record; records with iorb > Norb or more than
+ *   nstep steps are rejected (MOED_EARG). */
 int moed_eigvec_filename(int32_t isector, int32_t iev, int32_t node, char* buf, int32_t len);
 int moed_export_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t nloc, const double* vec);
 int moed_import_eigvec(const char* dir, int32_t isector, int32_t iev, int32_t node, int64_t* nloc, double* vec);

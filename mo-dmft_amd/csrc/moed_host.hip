@@ -1502,7 +1502,8 @@ int moed_import_green(moed_handle* h, const char* path) {
         take(r.bp, r.nsp + 1);
         take(r.an, r.nsn + 1);
         take(r.bn, r.nsn + 1);
-        if (r.iorb < 1 || r.iorb > h->norb) {
+        // moed_green_record hands out nstep + 1 coefficients per array: longer records do not fit
+        if (r.iorb < 1 || r.iorb > h->norb || r.nsp > h->nstep || r.nsn > h->nstep) {
             ok = false;
             break;
         }
@@ -1514,7 +1515,7 @@ int moed_import_green(moed_handle* h, const char* path) {
         recs.push_back(std::move(r));
     }
     std::fclose(f);
-    if (!ok) return fail(h, MOED_EARG, "import_green: malformed apbpanbn record");
+    if (!ok) return fail(h, MOED_EARG, "import_green: malformed apbpanbn record, or orbital / steps beyond the handle's");
     h->records = std::move(recs);
     return MOED_OK;
 }

@@ -14,7 +14,7 @@ module moed_capi
     use iso_c_binding
     implicit none
 
-    integer(c_int32_t), parameter :: MOED_F_EXACT_CMPLX = 1, MOED_F_GF_REORTH = 2
+    integer(c_int32_t), parameter :: MOED_F_EXACT_CMPLX = 1, MOED_F_BATCHED = 8, MOED_F_FIT_TREE = 16
 
     type, bind(C) :: moed_params
         integer(c_int32_t) :: norb, nsite

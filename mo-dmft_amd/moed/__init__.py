@@ -27,8 +27,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmoed.so")
 
 MOED_F_EXACT_CMPLX = 1
-MOED_F_GF_REORTH = 2
-MOED_F_NO_TUNING = 4
 MOED_F_BATCHED = 8
 MOED_F_FIT_TREE = 16
 

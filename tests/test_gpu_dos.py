@@ -78,6 +78,14 @@ def test_apbpanbn_import_and_export_match_the_reference_bytes(tmp_path):
         out = tmp_path / "again.dat"
         s.export_green(str(out))
         assert out.read_bytes() == z["ab_bytes"].tobytes()
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, nstep=5) as s:  # records of up to 7 steps do not fit
+        with pytest.raises(Exception):
+            s.import_green(str(path))
+    bad = tmp_path / "truncated.dat"
+    bad.write_bytes(z["ab_bytes"].tobytes()[:-3])
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, nstep=10) as s:
+        with pytest.raises(Exception):
+            s.import_green(str(bad))
 
 
 def test_records_and_eigenvectors_round_trip_after_a_solve(tmp_path):
