@@ -134,6 +134,32 @@ def test_hv_14site_properties_and_oracle(configs):
     assert np.max(np.abs(Hx - yo)) <= 1e-14 * np.abs(yo).max()
 
 
+def test_hv_15site_tij_rows_and_properties(configs):
+    """the largest sector: 15 sites, 3 orbitals with impurity hopping, (7, 8), D = 41,409,225 (odd
+    nU = 6435): 400 rows against the oracle's rows of H (first and last rows, every row-block edge,
+    and a seeded sample), symmetry and linearity of the whole product"""
+    c = configs["ns15_fit"]
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        D = s.dim(nu, nd)
+        x = start_vector(3, D)
+        y = start_vector(4, D)
+        Hx = s.multiply_H(nu, nd, x)
+        Hy = s.multiply_H(nu, nd, y)
+        Hxy = s.multiply_H(nu, nd, 2.0 * x + 0.25 * y)
+    assert abs(np.dot(y, Hx) - np.dot(Hy, x)) <= 1e-12 * np.linalg.norm(Hx) * np.linalg.norm(y)
+    assert np.max(np.abs(Hxy - (2.0 * Hx + 0.25 * Hy))) <= 1e-12 * np.abs(Hx).max()
+    m = omodel(c.norb, c.nbath, c.U, c.J, c.ek, c.vk, c.tij)
+    b = oracle.Basis(c.nsite, nu, nd)
+    nU = b.ntot // oracle.icom(c.nsite, nd)
+    rng = np.random.default_rng(15)
+    rows = np.unique(np.concatenate([[0, 1, D - 2, D - 1], np.arange(0, D, nU)[::97], np.arange(nU - 1, D, nU)[::97],
+                                     rng.integers(0, D, 400)]))[:400]
+    scale = np.abs(Hx).max()
+    for r, (cols, vals) in zip(rows, oracle.h_rows(m, b, rows=rows)):
+        assert abs(Hx[r] - np.dot(vals, x[cols])) <= 1e-14 * scale, r
+
+
 @pytest.mark.parametrize("case", [(2, 6, 4, 4), (3, 3, 3, 2), (1, 4, 0, 2), (2, 4, 6, 5)])
 def test_apply_c_bit_exact(case):
     norb, nbath, nu, nd = case

@@ -118,6 +118,38 @@ def test_solve_all_sectors_small_model_and_selection(flags):
         assert k[2:] == r[2:] and abs(k[0] - r[0]) < 1e-10 and abs(k[1] - r[1]) < 1e-9
 
 
+def test_four_orbitals_with_tij_against_dense_ed():
+    """Norb = 4 (the largest) with impurity hopping t_ij and two bath sites: the lowest eigenvalues
+    of four sectors against dense ED, and G(iw) of every orbital against the Lehmann sum"""
+    rng = np.random.default_rng(44)
+    norb, nbath = 4, 2
+    ns = norb + nbath
+    ek = rng.uniform(-1.5, 1.5, ns)
+    vk = np.zeros((norb, nbath))
+    vk[0, 0], vk[1, 1], vk[2, 0], vk[3, 1] = rng.uniform(0.2, 0.8, 4)
+    t = rng.uniform(-0.3, 0.3, (norb, norb))
+    t = t + t.T
+    np.fill_diagonal(t, 0.0)
+    U, J = 2.5, 0.3
+    sectors = [(3, 3), (2, 3), (3, 2), (4, 2)]
+    om = omega_grid(64, 40.0)
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=1000.0, nev=3, nstep=120, sectors=sectors, tij=t) as s:
+        s.ed_solve()
+        evs = [s.sector_eigenvalues(i + 1) for i in range(len(sectors))]
+    for (nu, nd), ev in zip(sectors, evs):
+        w = np.linalg.eigvalsh(dense_ed.hamiltonian(dense_ed.Sector(ns, nu, nd), norb, nbath, U, J, ek, vk, t).toarray())
+        assert np.allclose(ev[:3], w[:3], rtol=0, atol=1e-10 * max(1, abs(w[0])))
+    # G of the (3, 3) ground state (spin multiplets span sectors, so one sector keeps one state)
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=1000.0, nev=1, nstep=120, sectors=[(3, 3)], tij=t,
+                  flags=MOED_F_EXACT_CMPLX) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+    assert len(kept) == 1
+    for iorb in range(1, norb + 1):
+        _, g = dense_ed.lehmann_green(norb, nbath, U, J, ek, vk, 3, 3, iorb - 1, 0, om, tij=t)
+        assert np.max(np.abs(G[iorb - 1] - g)) <= 1e-9 * np.max(np.abs(g)), iorb
+
+
 def test_batched_solve_and_green_equal_sector_by_sector():
     """7-site model, all 64 sectors (dims up to 1225): the multi-problem run gives the same
     eigenvalues, kept states and G(iw) as the sector-by-sector run, bit for bit (every problem
