@@ -73,6 +73,9 @@ struct FitSmem {
     double bcast[2];
     double red[kFitThreads / 64];
     double gred[kFitThreads / 64][kFitCols];      // per-wave column sums (tree gradient)
+    // tree order: the nonzero couplings of each orbital at the point xt (V^2, V, e_k, k)
+    double nzv2[kMaxOrb][kFitMaxBath], nzv[kMaxOrb][kFitMaxBath], nzek[kMaxOrb][kFitMaxBath];
+    int32_t nzk[kMaxOrb][kFitMaxBath], nnz[kMaxOrb];
     union {
         double fstage[2][kFitChunkF * kMaxOrb];  // FUNC terms, two chunks (reference order)
         double cols[kFitCols * kFitChunkG];      // gradient terms of one chunk (reference order)
@@ -147,10 +150,38 @@ __device__ __forceinline__ double chain_sum(double s, const double* buf, int n) 
 // Tree order restates the terms with fewer divisions (each term then differs from the reference's
 // by a few ulps): 1/(i w - e_k) = (-e_k - i w) / (e_k^2 + w^2) with one division, 1/w_n once per
 // frequency, |r|^2 instead of hypot(r)^2.
+// 1/d for d > 0: v_rcp_f64 (about 2^-26 relative) and two Newton steps (the error squares each
+// step: rounding level, within an ulp of the division)
+__device__ __forceinline__ double fast_rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    return r;
+}
+
 __device__ __forceinline__ cxd inv_den(double ek, double zi) {
 #pragma clang fp contract(off)
-    const double c = 0.0 - ek, inv = 1.0 / (c * c + zi * zi);
+    const double c = 0.0 - ek, inv = fast_rcp(c * c + zi * zi);
     return {c * inv, -(zi * inv)};
+}
+
+// the nonzero couplings of every orbital at xt, in k order (a thread per orbital); ends with a barrier
+__device__ __forceinline__ void nonzero_couplings(const FitView& q) {
+    const int o = threadIdx.x;
+    if (o < q.norb) {
+        int n = 0;
+        for (int k = 0; k < q.nbath; ++k) {
+            const double v = S.xt[q.nbath + o * q.nbath + k];
+            if (v == 0.0) continue;
+            S.nzv2[o][n] = v * v;
+            S.nzv[o][n] = v;
+            S.nzek[o][n] = S.xt[k];
+            S.nzk[o][n] = k;
+            ++n;
+        }
+        S.nnz[o] = n;
+    }
+    __syncthreads();
 }
 
 // A thread's group of kGrp frequencies iw = base + tid + j kFitThreads.  The k loop runs outside
@@ -185,10 +216,9 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
     GroupResidual r;
 #pragma unroll
     for (int j = 0; j < kGrp; ++j) r.re[j] = r.im[j] = 0.0;
-    for (int k = 0; k < q.nbath; ++k) {
-        const double v = xt[q.nbath + o * q.nbath + k];
-        if (v == 0.0) continue;
-        const double ek = xt[k], v2 = v * v;
+    const int nn = uniform_i32(S.nnz[o]);
+    for (int n = 0; n < nn; ++n) {
+        const double ek = S.nzek[o][n], v2 = S.nzv2[o][n];
 #pragma unroll
         for (int j = 0; j < kGrp; ++j) {
             const cxd d = inv_den(ek, g.zi[j]);
@@ -216,6 +246,7 @@ __device__ __noinline__ double eval_func_tree(const FitProb qa) {
     const FitView q = view(qa);
     const int tid = threadIdx.x;
     double s = 0.0;
+    nonzero_couplings(q);
     {
         for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
             const FreqGroup g = freq_group(q, base);
@@ -306,6 +337,7 @@ __device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
     const int tid = threadIdx.x, nb = q.nbath, lane = tid & 63, wid = tid >> 6;
     const double scale1 = (double)(float)(q.norb * q.norb), scale2 = (double)(float)q.nw;
     for (int c = tid; c < q.nx; c += kFitThreads) S.gacc[c] = 0.0;
+    nonzero_couplings(q);
     // with d = 1/(i w - e_k): d/de_k 2 Re(conj(r) V^2 d^2)/w, d/dV 4 Re(conj(r) V d)/w, d/def 2 Re(r)/w.
     // Per (orbital, frequency group): each column is summed over the thread's frequencies, over the
     // wave, then over the waves in index order, and added to the running total.
@@ -313,10 +345,10 @@ __device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
         for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
             const FreqGroup g = freq_group(q, base);
             const GroupResidual r = residual_group(q, S.xt, o, g);
-            for (int k = 0; k < nb; ++k) {
-                const double v = S.xt[nb + o * nb + k];
-                if (v == 0.0) continue;
-                const double ek = S.xt[k], v2 = v * v;
+            const int nn = uniform_i32(S.nnz[o]);
+            for (int n = 0; n < nn; ++n) {
+                const int k = S.nzk[o][n];
+                const double ek = S.nzek[o][n], v2 = S.nzv2[o][n], v = S.nzv[o][n];
                 double pde = 0.0, pdv = 0.0;
 #pragma unroll
                 for (int j = 0; j < kGrp; ++j) {
