@@ -5,9 +5,12 @@ import this package, and only as the checker.  The product library never calls i
 
 Contents
   * ``liboracle.so`` (oracle.c): a plain-C restatement of the reference hot path
-    (ed_basis.F90, ed_operators.F90, ed_hamiltonian.F90:88-191, ed_lanczos.F90:14-68,
-    ed_solver.F90:193-219, ed_green.F90:25-185,260-273, func.f90, frprmn.f90), wrapped
-    here with ctypes.
+    (ed_basis.F90, ed_operators.F90, ed_hamiltonian.F90:29-85,88-191, ed_lanczos.F90:14-68,
+    ed_solver.F90:193-219, ed_green.F90:25-253,260-273, func.f90, frprmn.f90, dos.f90:65-106),
+    wrapped here with ctypes; ``dos`` is the record loop of dos.f90:1-63 in Python.
+    Every function is pinned by the tests in tests/test_oracle.py (reference-build outputs,
+    dense ED, closed forms); PARPACK, which the reference's diag calls, is absent and is not
+    restated (E0 is pinned by dense ED and SciPy's ARPACK instead).
   * ``dense_ed`` : an independent brute-force second-quantisation build of the same
     Hamiltonian (numpy/scipy), used to pin the restatement.
   * ``run_ref``  : runs the reference build ``oracle/_ref/ref_driver`` (the reference's own

@@ -18,9 +18,10 @@
 !   6 SORT        (func.f90:306)            i1=n, payload a(n) -> indx(n)
 !   7 H columns   (multiply_H on unit vectors) -> for each column: nnz, rows(nnz), vals(nnz)
 !   8 timing      i1 = calls; every MPI rank owns its row block -> seconds per call, ranks, y(1)
-!   9 ed_delta_new (ed_green.F90:187) after ed_set_band_structure (ed_hamiltonian.F90:29)
-!                 i1 = nw, i2 = Nq; payload rMu, Gr(2*norb*nw), D_ev(2*norb*nw)
-!                 -> Hk(i,i,q) real parts (norb*Nq), Gksum(2*norb*nw), D_ev(2*norb*nw)
+!   9 ed_delta_new (ed_green.F90:187) after ed_set_band_structure (ed_hamiltonian.F90:29), then
+!                 n_from_gksum (ed_green.F90:226)
+!                 i1 = nw, i2 = Nq; payload rMu, beta, Gr(2*norb*nw), D_ev(2*norb*nw)
+!                 -> Hk(i,i,q) real parts (norb*Nq), Gksum(2*norb*nw), D_ev(2*norb*nw), nocc(norb)
 !  10 dos_diag     (dos.f90:65)            i1 = nw, i2 = nstep_calcp, i3 = nstep_calcn; payload small, ev,
 !                 int32 even, omega_real(nw), ap, bp (0:nsp), an, bn (0:nsn) -> Aff(1, 1:nw) for iorb = 1
 !  11 export_eigvec (ed_io.F90:13)          i1 = isector, i2 = iev, i3 = nloc; payload eigvec(nloc)
@@ -35,7 +36,7 @@ program ref_driver
     use ed_operators
     use ed_hamiltonian
     use ed_lanczos
-    use ed_green, only: nwloc, Gr, D_ev, Gksum, ed_delta_new
+    use ed_green, only: nwloc, Gr, D_ev, Gksum, ed_delta_new, n_from_gksum, nlocals_w, offsets_w, nocc
     use ed_io, only: export_eigvec
     implicit none
     character(len=512) :: fin, fout
@@ -163,17 +164,23 @@ program ref_driver
     case (9)
         nwloc = i1
         Nq = i2
-        read(10) uj(1)
+        read(10) uj
         rMu = uj(1)
+        beta = uj(2)
         call ed_hamiltonian_init
         call ed_set_band_structure
         allocate(Gr(norb, nwloc), D_ev(norb, nwloc), Gksum(norb, nwloc))
         read(10) Gr
         read(10) D_ev
         call ed_delta_new
+        allocate(nlocals_w(0:0), offsets_w(0:0), nocc(norb))
+        nlocals_w(0) = nwloc
+        offsets_w(0) = 0
+        call n_from_gksum
         write(11) ((dble(Hk(i, i, j)), i = 1, norb), j = 1, Nq)
         write(11) Gksum
         write(11) D_ev
+        write(11) nocc
     case (10)
         nw = i1
         read(10) uj

@@ -137,14 +137,16 @@ def gen_ref_delta():
     Gr = 1.0 / (1j * om[None, :] - eps) + 0.05 * (rng.standard_normal((norb, nw)) + 1j * rng.standard_normal((norb, nw)))
     Dev = 0.3 * (rng.standard_normal((norb, nw)) - 1j * np.abs(rng.standard_normal((norb, nw))))
     m = oracle.Model(norb, 12, 4.0, 0.6, np.zeros(norb + 12), np.zeros((norb, 12)))
-    raw = oracle.run_ref(9, m, i1=nw, i2=nq, payload=[np.array([rmu]), oracle._dev_flat(Gr), oracle._dev_flat(Dev)])
+    beta = 200.0
+    raw = oracle.run_ref(9, m, i1=nw, i2=nq, payload=[np.array([rmu, beta]), oracle._dev_flat(Gr), oracle._dev_flat(Dev)])
     a = np.frombuffer(raw, "<f8")
     tocx = lambda v: (v[0::2] + 1j * v[1::2]).reshape(norb, nw, order="F")
     hk = a[:norb * nq].reshape(nq, norb)
     gks = tocx(a[norb * nq:norb * nq + 2 * norb * nw])
-    dev_new = tocx(a[norb * nq + 2 * norb * nw:])
-    np.savez_compressed(os.path.join(HERE, "ref_delta.npz"), norb=norb, nw=nw, nq=nq, rmu=rmu, Gr=Gr, Dev=Dev,
-                        hk=hk, gksum=gks, dev_new=dev_new)
+    dev_new = tocx(a[norb * nq + 2 * norb * nw:norb * nq + 4 * norb * nw])
+    nocc = a[norb * nq + 4 * norb * nw:].copy()
+    np.savez_compressed(os.path.join(HERE, "ref_delta.npz"), norb=norb, nw=nw, nq=nq, rmu=rmu, beta=beta, Gr=Gr,
+                        Dev=Dev, hk=hk, gksum=gks, dev_new=dev_new, nocc=nocc)
     print("  reference lattice-step fixture written")
 
 

@@ -167,12 +167,13 @@ def test_func_dfunc_frprmn_match_reference_build():
 
 
 def test_delta_new_matches_reference_build():
-    """lattice step (band structure, k-sum, new hybridisation) against the reference build"""
+    """lattice step (band structure, k-sum, new hybridisation, occupations) against the reference build"""
     z = np.load(os.path.join(GOLD, "ref_delta.npz"))
     hk = oracle.band_tb(int(z["norb"]), int(z["nq"]))
     assert np.array_equal(hk, z["hk"])
     gks, dn = oracle.delta_new(z["Gr"], z["Dev"], hk, float(z["rmu"]))
     assert np.array_equal(gks, z["gksum"]) and np.array_equal(dn, z["dev_new"])
+    assert np.array_equal(oracle.n_from_gksum(gks, float(z["beta"])), z["nocc"])  # n_from_gksum
 
 
 def test_delta_new_closed_forms():
