@@ -29,6 +29,9 @@
 !  12 apbpanbn.dat records as ed_calc_green_ftn writes them (ed_green.F90:75, 112-114): i1 = number
 !                 of records; per record payload int32 nsp, nsn, iev, iorb, even; real64 val, prob,
 !                 ap, bp (0:nsp), an, bn (0:nsn) -> size in bytes (int32), the file's bytes
+!  13 dos (dos.f90:1-63) reading such records: i1 = number of records, i2 = nw, i3 = nev_calc;
+!                 payload as task 12, then real64 (small, Nstep), omega_real(nw), and (val, prob) of
+!                 the nev_calc states -> Aff(norb, nw); Nstep bounds every record's step counts
 program ref_driver
     use parallel_params
     use ed_config
@@ -38,6 +41,7 @@ program ref_driver
     use ed_lanczos
     use ed_green, only: nwloc, Gr, D_ev, Gksum, ed_delta_new, n_from_gksum, nlocals_w, offsets_w, nocc
     use ed_io, only: export_eigvec
+    use ed_solver, only: eigval, nev_calc
     implicit none
     character(len=512) :: fin, fout
     integer :: hdr(8), task, ne_up, ne_dn, i1, i2, i3, i, j, nnz, nstep_calc, nx, nwl
@@ -203,8 +207,37 @@ program ref_driver
         write(fname, "(A7,I2.2,A1,I3.3,A1,I4.4)") "eigvec-", i1, "-", i2, "-", 0
         call copy_file(trim(fname))
     case (12)
+        call write_records(i1)
+        fname = "apbpanbn.dat"
+        call copy_file("apbpanbn.dat")
+    case (13)
+        call write_records(i1)
+        nw = i2
+        read(10) uj
+        small = uj(1)
+        Nstep = nint(uj(2))
+        allocate(om(nw), aff(norb, nw))
+        read(10) om
+        nev_calc = i3
+        allocate(eigval(nev_calc))
+        do j = 1, nev_calc
+            read(10) uj
+            eigval(j)%val = uj(1)
+            eigval(j)%prob = uj(2)
+        enddo
+        call dos(om, aff)
+        write(11) aff
+    end select
+    if (node .eq. 0) close(11)
+    close(10)
+    call MPI_Finalize(ierr)
+contains
+    ! apbpanbn.dat with ed_calc_green_ftn's record statements (ed_green.F90:75, 112-114)
+    subroutine write_records(nrec)
+        integer, intent(in) :: nrec
+        integer :: r
         open(89, file="apbpanbn.dat", form="unformatted", status="new")
-        do j = 1, i1
+        do r = 1, nrec
             read(10) rhdr
             even = rhdr(5) .ne. 0
             allocate(a(2), ap(0:rhdr(1)), bp(0:rhdr(1)), an(0:rhdr(2)), bn(0:rhdr(2)))
@@ -219,13 +252,8 @@ program ref_driver
             deallocate(a, ap, bp, an, bn)
         enddo
         close(89)
-        fname = "apbpanbn.dat"
-        call copy_file("apbpanbn.dat")
-    end select
-    if (node .eq. 0) close(11)
-    close(10)
-    call MPI_Finalize(ierr)
-contains
+    end subroutine write_records
+
     ! the bytes of a file written by the reference's I/O, appended to the output
     subroutine copy_file(name)
         character(len=*), intent(in) :: name

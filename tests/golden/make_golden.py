@@ -196,9 +196,17 @@ def gen_ref_files():
                      f"rec{i}_ap": ab[0], f"rec{i}_bp": ab[1], f"rec{i}_an": ab[2], f"rec{i}_bn": ab[3]})
     raw = oracle.run_ref(12, m, i1=len(recs), payload=payload)
     n = struct.unpack("<i", raw[150:154])[0]
+    ab_bytes = np.frombuffer(raw[154:154 + n], np.uint8).copy()
+    # the reference's dos (dos.f90:1-63) reading these records: Aff(norb = 2, nw)
+    nw, small = 200, 0.05
+    om = (np.arange(1, nw + 1) - nw // 2) * 0.01
+    states = [np.array([-3.25, 0.625]), np.array([-1.5, 0.1875])]
+    m2 = oracle.Model(2, 1, 0.0, 0.0, np.zeros(3), np.zeros((2, 1)))
+    raw = oracle.run_ref(13, m2, i1=len(recs), i2=nw, i3=len(states), payload=payload + [np.array([small, 50.0]), om] + states)
+    dos_aff = np.frombuffer(raw, "<f8").reshape(2, nw, order="F").copy()
     np.savez_compressed(os.path.join(HERE, "ref_files.npz"), vec=vec, eig_name=eig_name, eig_bytes=eig_bytes,
-                        eig_name_121=eig_name_121, nrec=len(recs),
-                        ab_bytes=np.frombuffer(raw[154:154 + n], np.uint8).copy(), **flat)
+                        eig_name_121=eig_name_121, nrec=len(recs), ab_bytes=ab_bytes, dos_omega=om, dos_small=small,
+                        dos_aff=dos_aff, **flat)
     print("  reference on-disk record fixture written")
 
 

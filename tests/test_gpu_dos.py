@@ -88,6 +88,19 @@ def test_apbpanbn_import_and_export_match_the_reference_bytes(tmp_path):
             s.import_green(str(bad))
 
 
+def test_dos_of_imported_records_matches_reference_build(tmp_path):
+    """moed_import_green of the reference-written apbpanbn.dat, then moed_dos: Aff equals the
+    reference build's dos (dos.f90:1-63) over the same file, bit for bit"""
+    z = np.load(os.path.join(GOLD, "ref_files.npz"))
+    path = tmp_path / "apbpanbn.dat"
+    path.write_bytes(z["ab_bytes"].tobytes())
+    norb, nbath, ek, vk = _model()
+    with EDSolver(norb, nbath, 3.0, 0.4, ek, vk, nstep=10) as s:
+        s.import_green(str(path))
+        a = s.dos(z["dos_omega"], float(z["dos_small"]))
+        assert np.array_equal(a, z["dos_aff"])
+
+
 def test_records_and_eigenvectors_round_trip_after_a_solve(tmp_path):
     """export after ed_solve / ed_calc_green_ftn, import into a fresh solver: the same records and
     the same spectral function; every eigenvector file reads back as the device's eigenvector"""

@@ -349,6 +349,27 @@ def test_dos_diag_single_level_closed_form():
     assert abs(np.trapezoid(aw, wide) - (bp + bn)) <= 1e-3
 
 
+def _ref_file_records(z):
+    recs = []
+    for i in range(int(z["nrec"])):
+        nsp, nsn, iev, iorb, even = (int(v) for v in z[f"rec{i}_hdr"])
+        recs.append(dict(iev=iev, iorb=iorb, E=z[f"rec{i}_vp"][0], p=z[f"rec{i}_vp"][1], even=bool(even), nsp=nsp,
+                         nsn=nsn, ap=z[f"rec{i}_ap"], bp=z[f"rec{i}_bp"], an=z[f"rec{i}_an"], bn=z[f"rec{i}_bn"]))
+    return recs
+
+
+def test_dos_over_records_matches_reference_build():
+    """dos (dos.f90:1-63) of the reference build reading an apbpanbn.dat of six records (one state
+    with N_up = N_dn, one with both spins, two orbitals) against the oracle's record loop, bit for
+    bit; the result is the last record's row per orbital, not an earlier one's"""
+    z = np.load(os.path.join(GOLD, "ref_files.npz"))
+    recs = _ref_file_records(z)
+    om, small = z["dos_omega"], float(z["dos_small"])
+    aff = oracle.dos(recs, 2, om, small)
+    assert np.array_equal(aff, z["dos_aff"])
+    assert not np.allclose(oracle.dos(recs[:4], 2, om, small), z["dos_aff"])
+
+
 def test_dos_records_overwrite_and_weighted():
     """dos.f90:17-59: every record overwrites its orbital's row; the weighted variant sums
     p * factor * A (factor 1/2 when N_up != N_dn)"""
