@@ -71,10 +71,10 @@ struct StrTable {
     int n = 0, count = 0, mh = 0, nslot = 0, nq = 0;
     std::vector<uint32_t> str;
     std::vector<int32_t> rank;  // [2^Ns]: rank within this popcount class, -1 otherwise
-    DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec;
+    DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec, d_rec8;
     StrDev dev() const {
         return {d_str.as<uint16_t>(), d_nhop.as<int32_t>(), d_mv.as<uint32_t>(), d_mvc.as<double>(),
-                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    count, mh, nslot, nq};
+                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    d_rec8.as<uint4>(),  count, mh, nslot, nq, n};
     }
 };
 
@@ -302,7 +302,9 @@ int build_table(moed_handle* h, int n, StrTable** out) {
     }
     // row-pass records (moed_internal.h, StrDev::rec): onsite prefix, string, slot entries
     t->nslot = (int)h->slots.size();
-    t->nq = (10 + 2 * t->nslot + 15) / 16;
+    // 16-byte units per record: room for the onsite prefix, the string and the slots, and for the
+    // record_dwords(nslot) dwords the row kernels read (their last slots are padding entries)
+    t->nq = std::max((10 + 2 * t->nslot + 15) / 16, (record_dwords(t->nslot) + 3) / 4);
     const size_t ru16 = (size_t)t->nq * 8;  // uint16 per record
     std::vector<uint16_t> rec(ru16 * C, (uint16_t)C);
     for (int r = 0; r < C; ++r) {
@@ -321,6 +323,12 @@ int build_table(moed_handle* h, int n, StrTable** out) {
     }
     HIPCHK(t->d_rec.ensure(rec.size() * sizeof(uint16_t)));
     HIPCHK(hipMemcpy(t->d_rec.p, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    if (8 * (2 * (int64_t)C + 1) < 65536) {  // the same records with slot entries as LDS byte offsets
+        for (int r = 0; r < C; ++r)  // every slot position, the padding slots' zero entries included
+            for (size_t k = 5; k < ru16; ++k) rec[ru16 * r + k] = (uint16_t)(8 * rec[ru16 * r + k]);
+        HIPCHK(t->d_rec8.ensure(rec.size() * sizeof(uint16_t)));
+        HIPCHK(hipMemcpy(t->d_rec8.p, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     str16.resize(C + 2, 0);  // scalar reads fetch whole 4-byte words
     HIPCHK(t->d_str.ensure((C + 2) * sizeof(uint16_t)));
     HIPCHK(t->d_nhop.ensure(C * sizeof(int32_t)));

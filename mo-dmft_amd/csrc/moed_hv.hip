@@ -21,6 +21,8 @@
 // sum bit-identical (the sum starts at +0.0 and can never become -0.0).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "moed_device.h"
 #include "moed_hvops.h"
 #include "moed_internal.h"
@@ -35,12 +37,24 @@ namespace moed {
 // coefficient, and slots that do not apply (and the padding slots past nslot, coefficient 0.0)
 // read the zero: the KMAX = 2 NW - 5 slot reads carry no branches and are all in flight
 // together.  One column per thread and iteration; the column's record (onsite prefix, string,
-// slot entries) is read from global memory (L2-resident, shared by all rows).
-template <int NORB, int NW>
+// slot entries) is read from global memory (L2-resident, shared by all rows), one column ahead
+// (the first before the row is staged), so that each wait for a record overlaps a whole column's
+// work and the previous column's store.
+//   BYTES   the up strings have byte-offset records (nU <= kFastMaxU): the staged row is then the
+//           kernel's only LDS object, at LDS address 0, and a slot entry (8 x the entry) is the
+//           address it reads, which saves the index arithmetic per slot
+//   NDP     the energies of the row's occupied down sites sit in scalar registers, padded with +0.0
+//           to NDP adds (S + 0.0 == S: the sum is never -0.0)
+// The interaction terms use the row's RowDiag coefficients.  The diagonal is diag_element's, term
+// for term.
+constexpr int kFastMaxU = 4095;
+
+template <int NORB, int NW, int NDP, bool BYTES>
 __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     constexpr int KMAX = 2 * NW - 5;
-    extern __shared__ double xs[];
-    __shared__ double ekd[kMaxSite];
+    __shared__ double xstat[BYTES ? 2 * kFastMaxU + 2 : 1];
+    extern __shared__ double xdyn[];
+    double* const xs = BYTES ? xstat : xdyn;
 
     if (P.stop && *P.stop) return;
     const int NT = blockDim.x;
@@ -50,6 +64,13 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     const int tid = threadIdx.x;
     const double* __restrict__ xr = P.x + (size_t)id * nU;
     double* __restrict__ yr = P.y + (size_t)id * nU;
+    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(BYTES ? P.up.rec8 : P.up.rec);
+    const int rstride = 4 * P.up.nq;
+    uint32_t wn[NW];
+    if (tid < nU) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) wn[q] = recw[(size_t)tid * rstride + q];
+    }
 
     // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned)
     if ((nU & 1) == 0) {
@@ -66,29 +87,45 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
             xs[neg + k] = -v;
         }
     }
+    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
     const uint32_t sd = scalar_u16(P.dn.str, id);
-    const int nd = __builtin_popcount(sd);
+    double ekr[NDP];
+    {
+        const cptr<double> ekc = as_const(P.ek);
+        uint32_t m = sd;
+#pragma unroll
+        for (int q = 0; q < NDP; ++q) {
+            const double e = ekc[m ? __builtin_ctz(m) : 0];
+            ekr[q] = m ? e : 0.0;
+            m &= m - 1;
+        }
+    }
+    const RowDiag<NORB> rd = row_diag<NORB>(sd, U, UmJ3, Um2J);
     if (tid == 0) {
         xs[nU] = 0.0;
         xs[2 * nU + 1] = 0.0;
-        const cptr<double> ekc = as_const(P.ek);
-        uint32_t m = sd;
-        for (int q = 0; m; m &= m - 1, ++q) ekd[q] = ekc[__builtin_ctz(m)];
     }
     __syncthreads();
 
-    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
     const cptr<double> slotc = as_const(P.slotc);
-    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(P.up.rec);
-    const int rstride = 4 * P.up.nq;
     for (int iu = tid; iu < nU; iu += NT) {
         uint32_t w[NW];
 #pragma unroll
-        for (int q = 0; q < NW; ++q) w[q] = recw[(size_t)iu * rstride + q];
-        const double Sup = __hiloint2double((int)w[1], (int)w[0]);
-        double acc = __dmul_rn(diag_element<NORB>(Sup, w[2] & 0xffffu, sd, ekd, nd, U, UmJ3, Um2J), xs[iu]);
+        for (int q = 0; q < NW; ++q) w[q] = wn[q];
+        if (iu + NT < nU) {
 #pragma unroll
-        for (int q = 0; q < KMAX; ++q) acc = fma(slotc[q], xs[rec_u16(w, 5 + q)], acc);
+            for (int q = 0; q < NW; ++q) wn[q] = recw[(size_t)(iu + NT) * rstride + q];
+        }
+        double S = __hiloint2double((int)w[1], (int)w[0]);
+#pragma unroll
+        for (int q = 0; q < NDP; ++q) S += ekr[q];
+        double acc = __dmul_rn(diag_interaction<NORB>(S, w[2] & 0xffffu, rd, UmJ3), xs[iu]);
+#pragma unroll
+        for (int q = 0; q < KMAX; ++q) {
+            const uint32_t e = rec_u16(w, 5 + q);
+            const double v = BYTES ? *reinterpret_cast<const double*>(reinterpret_cast<const char*>(xstat) + e) : xdyn[e];
+            acc = fma(slotc[q], v, acc);
+        }
         yr[iu] = acc;
     }
 }
@@ -284,7 +321,9 @@ HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu) {
     (void)ncu;
     if (strip <= 0) strip = 128;
     p.cpl = (strip >= 128 && (nU % 2) == 0) ? 2 : 1;
-    p.rows_b = 4 * (kStripThreads / 64);
+    // rows per work item: 8 per wave with 128-column strips, 32 with 64-column strips (items of
+    // 1-2 K elements; measured best at the 14-site (7, 7) and 15-site (7, 8) sectors)
+    p.rows_b = (p.cpl == 2 ? 2 : 8) * (kStripThreads / 64);
     const int64_t w = 64 * p.cpl;
     const int64_t nstrip = (nU + w - 1) / w;
     const int64_t bps = (nD + p.rows_b - 1) / p.rows_b;
@@ -293,16 +332,27 @@ HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu) {
     return p;
 }
 
-template <int NORB, int NW>
-static hipError_t rows_launch_t(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+template <int NORB, int NW, int NDP>
+static hipError_t rows_launch_nd(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+    if (a.up.rec8 && a.up.count <= kFastMaxU) {
+        hipLaunchKernelGGL((hv_rows_kernel<NORB, NW, NDP, true>), dim3(pl.grid_a), dim3(pl.threads_a), 0, s, a);
+        return hipGetLastError();
+    }
     const size_t lds = (2 * (size_t)a.up.count + 2) * sizeof(double);
-    auto kern = hv_rows_kernel<NORB, NW>;
+    auto kern = hv_rows_kernel<NORB, NW, NDP, false>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(pl.grid_a), dim3(pl.threads_a), lds, s, a);
     return hipGetLastError();
+}
+
+// down-site energies: 8 adds for up to 8 down electrons, else kMaxSite
+template <int NORB, int NW>
+static hipError_t rows_launch_t(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+    if (a.dn.nel < 0 || a.dn.nel > kMaxSite) return hipErrorInvalidValue;
+    return a.dn.nel <= 8 ? rows_launch_nd<NORB, NW, 8>(a, pl, s) : rows_launch_nd<NORB, NW, kMaxSite>(a, pl, s);
 }
 
 template <int NORB>

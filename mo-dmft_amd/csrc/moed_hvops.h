@@ -35,6 +35,51 @@ __device__ __forceinline__ double diag_element(double Sup, uint32_t su, uint32_t
     return S;
 }
 
+// The interaction part of diag_element with the down string's conditions turned into per-row
+// coefficients: a term whose down orbital is empty is added with coefficient 0.0 instead of being
+// skipped.  fma(n, 0.0, S) and S + 0.0 equal S exactly (S is never -0.0), so the result is
+// bit-identical to diag_element's, with no per-element selects.
+template <int NORB>
+struct RowDiag {
+    double Uo[NORB];         // U where down orbital o is occupied, else 0.0
+    double A[NORB];          // U - 2J where down orbital o is occupied, else 0.0
+    double B[NORB][NORB];    // U - 3J where down orbitals o and p are both occupied, else 0.0
+};
+
+template <int NORB>
+__device__ __forceinline__ RowDiag<NORB> row_diag(uint32_t sd, double U, double UmJ3, double Um2J) {
+    RowDiag<NORB> r;
+#pragma unroll
+    for (int o = 0; o < NORB; ++o) {
+        const bool d = (sd >> o) & 1u;
+        r.Uo[o] = d ? U : 0.0;
+        r.A[o] = d ? Um2J : 0.0;
+#pragma unroll
+        for (int p = 0; p < NORB; ++p) r.B[o][p] = (d && ((sd >> p) & 1u)) ? UmJ3 : 0.0;
+    }
+    return r;
+}
+
+// diag_element's interaction terms, in its order, on top of S = onsite sum of the up and down strings
+template <int NORB>
+__device__ __forceinline__ double diag_interaction(double S, uint32_t su, const RowDiag<NORB>& r, double UmJ3) {
+    double n[NORB];
+#pragma unroll
+    for (int o = 0; o < NORB; ++o) n[o] = (double)((su >> o) & 1u);
+#pragma unroll
+    for (int o = 0; o < NORB; ++o) S = fma(n[o], r.Uo[o], S);
+#pragma unroll
+    for (int o = 0; o < NORB; ++o)
+#pragma unroll
+        for (int p = o + 1; p < NORB; ++p) {
+            S = fma(n[o] * n[p], UmJ3, S);
+            S = fma(n[o], r.A[p], S);
+            S = fma(n[p], r.A[o], S);
+            S += r.B[o][p];
+        }
+    return S;
+}
+
 // uint16 number i of a record held in registers
 template <int NW>
 __device__ __forceinline__ uint32_t rec_u16(const uint32_t (&w)[NW], int i) {

@@ -42,10 +42,12 @@ struct StrDev {
     const double* mvc;
     const int32_t* imp;
     const uint4* rec;
+    const uint4* rec8;  // the records with slot entries as byte offsets (8 entry); nullptr unless 8 (2 count + 1) < 65536
     int32_t count;
     int32_t mh;
     int32_t nslot;
     int32_t nq;
+    int32_t nel;  // electrons per string
 };
 
 // Sector H v in two passes (moed_hv.hip).
