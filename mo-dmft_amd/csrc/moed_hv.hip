@@ -72,19 +72,40 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
         for (int q = 0; q < NW; ++q) wn[q] = recw[(size_t)tid * rstride + q];
     }
 
-    // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned)
+    // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned);
+    //      each thread issues all its loads of a round before it writes any of them to LDS
     if ((nU & 1) == 0) {
-        for (int p = tid; p < nU / 2; p += NT) {
-            const double2 v = reinterpret_cast<const double2*>(xr)[p];
-            reinterpret_cast<double2*>(xs)[p] = v;
-            xs[neg + 2 * p] = -v.x;
-            xs[neg + 2 * p + 1] = -v.y;
+        constexpr int R = 4;
+        for (int p0 = tid; p0 < nU / 2; p0 += R * NT) {
+            double2 v[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                if (p0 + u * NT < nU / 2) v[u] = reinterpret_cast<const double2*>(xr)[p0 + u * NT];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int p = p0 + u * NT;
+                if (p < nU / 2) {
+                    reinterpret_cast<double2*>(xs)[p] = v[u];
+                    xs[neg + 2 * p] = -v[u].x;
+                    xs[neg + 2 * p + 1] = -v[u].y;
+                }
+            }
         }
     } else {
-        for (int k = tid; k < nU; k += NT) {
-            const double v = xr[k];
-            xs[k] = v;
-            xs[neg + k] = -v;
+        constexpr int R = 8;
+        for (int k0 = tid; k0 < nU; k0 += R * NT) {
+            double v[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                if (k0 + u * NT < nU) v[u] = xr[k0 + u * NT];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int k = k0 + u * NT;
+                if (k < nU) {
+                    xs[k] = v[u];
+                    xs[neg + k] = -v[u];
+                }
+            }
         }
     }
     const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
