@@ -160,6 +160,41 @@ def test_hv_15site_tij_rows_and_properties(configs):
         assert abs(Hx[r] - np.dot(vals, x[cols])) <= 1e-14 * scale, r
 
 
+# move-slot counts whose records once had a stride shorter than the dwords the row pass reads
+# (2, 3, 10, 11, 28, 29 slots), small sectors on both sides of the byte-offset limit nU <= 4095,
+# and sectors with more than 8 down electrons (15 down-site adds)
+SLOT_CASES = [
+    # norb, nbath, dense V, t_ij, V entries kept (None: all), sectors, slots
+    (1, 2, True, False, None, [(1, 1), (2, 1)], 2),
+    (1, 3, True, False, None, [(2, 2), (1, 3)], 3),
+    (2, 8, False, False, None, [(3, 4)], 8),
+    (1, 11, True, False, None, [(3, 3), (6, 6)], 11),
+    (2, 13, True, True, None, [(1, 10), (7, 0), (2, 2), (7, 15)], 28),
+    (3, 12, True, True, 23, [(2, 2), (1, 9), (7, 15), (8, 1)], 29),
+]
+
+
+@pytest.mark.parametrize("case", SLOT_CASES)
+def test_hv_slot_counts_and_sector_shapes(case):
+    norb, nbath, dense, tij, keep, sectors, nslot = case
+    print("case", case, flush=True)
+    rng = np.random.default_rng(100 + nslot)
+    ek, vk, t = random_model(rng, norb, nbath, tij, dense)
+    if keep is not None:  # keep the first `keep` couplings of a seeded order
+        idx = rng.permutation(vk.size)[keep:]
+        vk.reshape(-1)[idx] = 0.0
+    assert np.count_nonzero(vk) + (0 if t is None else np.count_nonzero(t)) == nslot
+    m = omodel(norb, nbath, 2.5, 0.4, ek, vk, t)
+    with EDSolver(norb, nbath, 2.5, 0.4, ek, vk, tij=t) as s:
+        for nu, nd in sectors:
+            D = s.dim(nu, nd)
+            x = start_vector(nu * 17 + nd, D)
+            y = s.multiply_H(nu, nd, x)
+            yo = oracle.multiply_H(m, oracle.Basis(norb + nbath, nu, nd), x)
+            assert np.isfinite(y).all(), (nu, nd)
+            assert np.max(np.abs(y - yo)) <= 1e-14 * np.abs(yo).max(), (nu, nd)
+
+
 @pytest.mark.parametrize("case", [(2, 6, 4, 4), (3, 3, 3, 2), (1, 4, 0, 2), (2, 4, 6, 5)])
 def test_apply_c_bit_exact(case):
     norb, nbath, nu, nd = case
