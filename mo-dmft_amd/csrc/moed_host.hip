@@ -554,7 +554,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         if (rc) return rc;
         dim[p] = S[p].D;
         off[p] = dtot;
-        dtot += dim[p];
+        dtot += dim[p] + (dim[p] & 1);  // every problem starts on an even element: the H v strip pass
+                                        // reads 16-byte pairs (the gap element is never touched)
         mmax[p] = (int32_t)std::min<int64_t>(h->gs_maxvec, dim[p]);
         M = std::max(M, (int)mmax[p]);
     }

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
 
     // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned);
     //      each thread issues all its loads of a round before it writes any of them to LDS
-    if ((nU & 1) == 0) {
+    if ((nU & 1) == 0 && (reinterpret_cast<uintptr_t>(P.x) & 15) == 0) {
         constexpr int R = 4;
         for (int p0 = tid; p0 < nU / 2; p0 += R * NT) {
             double2 v[R];
@@ -408,7 +408,16 @@ static hipError_t hv_launch_norb(const HvArgs& a0, const HvPlan& pl, hipStream_t
     a.items_per_xcd = pl.items_per_xcd;
     hipError_t e = rows_launch<NORB>(a, pl, s);
     if (e != hipSuccess) return e;
-    if (pl.cpl == 2)
+    const bool al16 = ((reinterpret_cast<uintptr_t>(a.x) | reinterpret_cast<uintptr_t>(a.y)) & 15) == 0;
+    if (pl.cpl == 2 && (a.up.count % 2) == 0 && !al16) {
+        // 16-byte pairs need 16-byte aligned x and y: a caller's misaligned vectors take the
+        // 64-column strips; the x.y partials are sized for the plan's strips, so none are asked for here
+        if (a.dot_part) return hipErrorInvalidValue;
+        const HvPlan p1 = hv_plan(a.up.count, a.dn.count, pl.threads_a, 64, 0);
+        a.rows_b = p1.rows_b;
+        a.items_per_xcd = p1.items_per_xcd;
+        hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(p1.grid_b), dim3(kStripThreads), 0, s, a);
+    } else if (pl.cpl == 2)
         hipLaunchKernelGGL((hv_strip_kernel<NORB, 2>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);
     else
         hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);

@@ -17,6 +17,10 @@ def omodel(norb, nbath, U, J, ek, vk, tij=None):
     return oracle.Model(norb, nbath, U, J, ek, vk, tij)
 
 
+def configs_by_name():
+    return {c.name: c for c in load_configs()}
+
+
 def random_model(rng, norb, nbath, tij=False, dense_v=False):
     ns = norb + nbath
     ek = rng.uniform(-2, 2, ns)
@@ -193,6 +197,36 @@ def test_hv_slot_counts_and_sector_shapes(case):
             yo = oracle.multiply_H(m, oracle.Basis(norb + nbath, nu, nd), x)
             assert np.isfinite(y).all(), (nu, nd)
             assert np.max(np.abs(y - yo)) <= 1e-14 * np.abs(yo).max(), (nu, nd)
+
+
+@pytest.mark.parametrize("name, sector", [("ns15_fit", (2, 2)), ("ns15_fit", (8, 1)), ("ns15_fit", (1, 14)),
+                                          ("ns15_fit", (6, 5)), ("ns12_plain", (6, 6)), ("ns12_plain", (5, 6))])
+def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
+    """moed_hv_device with x and y starting at either 8-byte phase (odd nU at 15 sites, where every
+    C(15, k) is odd, and even nU at 12 sites, whose 16-byte row and strip loads need 16-byte aligned
+    vectors and otherwise fall back to 8-byte loads): y is bit-identical to the host-buffer product
+    and to the 64-column (8-byte load) strips in every case, and nothing is written outside y"""
+    import torch
+    c = configs_by_name()[name]
+    nu, nd = sector
+    with EDSolver.from_config(c) as s:
+        D = s.dim(nu, nd)
+        x = start_vector(nu * 31 + nd, D)
+        ref = s.multiply_H(nu, nd, x)
+        for xoff in (0, 1):
+            for yoff in (0, 1):
+                print("case", name, sector, xoff, yoff, flush=True)
+                xb = torch.zeros(D + 2, dtype=torch.float64, device="cuda")
+                yb = torch.zeros(D + 2, dtype=torch.float64, device="cuda")
+                xb[xoff:xoff + D] = torch.from_numpy(x)
+                torch.cuda.synchronize()
+                s.multiply_H_device(nu, nd, xb.data_ptr() + 8 * xoff, yb.data_ptr() + 8 * yoff)
+                torch.cuda.synchronize()
+                y = yb.cpu().numpy()
+                assert np.array_equal(y[yoff:yoff + D], ref)
+                assert y[:yoff].sum() == 0.0 and y[yoff + D:].sum() == 0.0  # nothing written outside y
+        s.set_hv_shape(0, 64)
+        assert np.array_equal(s.multiply_H(nu, nd, x), ref)
 
 
 @pytest.mark.parametrize("case", [(2, 6, 4, 4), (3, 3, 3, 2), (1, 4, 0, 2), (2, 4, 6, 5)])
