@@ -186,15 +186,20 @@ __device__ __forceinline__ void nonzero_couplings(const FitView& q) {
 
 // A thread's group of kGrp frequencies iw = base + tid + j kFitThreads.  The k loop runs outside
 // the frequency loop, so V(o,k) and e_k are read and tested once per k and the kGrp divisions of
-// one k are independent of each other.
+// one k are independent of each other.  Slots j >= jn (block-uniform) lie past the last frequency
+// for every thread: the group helpers are instantiated per slot count JN = jn and skip them.  Their
+// terms are exactly +0.0 (zero residual, zero weight), so skipping them leaves every sum
+// bit-identical while a short frequency axis (Nw < kGrp kFitThreads) costs only the slots it fills.
 constexpr int kGrp = 4;
 struct FreqGroup {
     double zi[kGrp], rw[kGrp];  // Im z, 1/w_n (0 for a frequency past the end)
     int iw[kGrp];
+    int jn;                     // slots holding any frequency of the block
 };
 
 __device__ __forceinline__ FreqGroup freq_group(const FitView& q, int base) {
     FreqGroup g;
+    g.jn = min(kGrp, (q.nw - base + kFitThreads - 1) / kFitThreads);
 #pragma unroll
     for (int j = 0; j < kGrp; ++j) {
         const int iw = base + (int)threadIdx.x + j * kFitThreads;
@@ -211,6 +216,7 @@ __device__ __forceinline__ FreqGroup freq_group(const FitView& q, int base) {
 struct GroupResidual {
     double re[kGrp], im[kGrp];
 };
+template <int JN>
 __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const double* xt, int o,
                                                         const FreqGroup& g) {
     GroupResidual r;
@@ -220,7 +226,7 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
     for (int n = 0; n < nn; ++n) {
         const double ek = S.nzek[o][n], v2 = S.nzv2[o][n];
 #pragma unroll
-        for (int j = 0; j < kGrp; ++j) {
+        for (int j = 0; j < JN; ++j) {
             const cxd d = inv_den(ek, g.zi[j]);
             r.re[j] = fma(v2, d.re, r.re[j]);
             r.im[j] = fma(v2, d.im, r.im[j]);
@@ -228,7 +234,7 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
     }
     const double ef = xt[q.nx - q.norb + o];
 #pragma unroll
-    for (int j = 0; j < kGrp; ++j) {
+    for (int j = 0; j < JN; ++j) {
         const bool ok = g.iw[j] >= 0;
         const size_t io = (size_t)o + (size_t)q.norb * (ok ? g.iw[j] : 0);
         const double dre = q.dev[2 * io], dim = q.dev[2 * io + 1];
@@ -241,6 +247,17 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
 // FUNC (func.f90:30-42): sum over (w, o) of |r|^2 / w_n, divided by float(Norb*Norb*Nw).
 // Reference order: waves 1.. stage the terms of frequency chunk c while lane 0 of wave 0 adds chunk
 // c - 1 to the running sum (double-buffered).  Tree order: per-thread partials, then block_sum.
+template <int JN>
+__device__ __forceinline__ double func_group(const FitView& q, const FreqGroup& g, double s) {
+#pragma clang fp contract(off)
+    for (int o = 0; o < q.norb; ++o) {
+        const GroupResidual r = residual_group<JN>(q, S.xt, o, g);
+#pragma unroll
+        for (int j = 0; j < JN; ++j) s = fma(fma(r.re[j], r.re[j], r.im[j] * r.im[j]), g.rw[j], s);
+    }
+    return s;
+}
+
 __device__ __noinline__ double eval_func_tree(const FitProb qa) {
 #pragma clang fp contract(off)
     const FitView q = view(qa);
@@ -250,10 +267,11 @@ __device__ __noinline__ double eval_func_tree(const FitProb qa) {
     {
         for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
             const FreqGroup g = freq_group(q, base);
-            for (int o = 0; o < q.norb; ++o) {
-                const GroupResidual r = residual_group(q, S.xt, o, g);
-#pragma unroll
-                for (int j = 0; j < kGrp; ++j) s = fma(fma(r.re[j], r.re[j], r.im[j] * r.im[j]), g.rw[j], s);
+            switch (g.jn) {
+                case 4: s = func_group<4>(q, g, s); break;
+                case 3: s = func_group<3>(q, g, s); break;
+                case 2: s = func_group<2>(q, g, s); break;
+                default: s = func_group<1>(q, g, s); break;
             }
         }
         s = block_sum<kFitThreads>(s, S.red);
@@ -329,6 +347,39 @@ __device__ __forceinline__ void grad_terms(const FitView& q, const double* xt, i
     put(2 * kFitMaxBath, 2.0 * r.re / w);
 }
 
+// one (orbital, frequency group) of the tree-order dfunc: per-wave column sums into S.gred
+template <int JN>
+__device__ __forceinline__ void grad_group(const FitView& q, int o, const FreqGroup& g, int lane, int wid) {
+#pragma clang fp contract(off)
+    const GroupResidual r = residual_group<JN>(q, S.xt, o, g);
+    const int nn = uniform_i32(S.nnz[o]);
+    for (int n = 0; n < nn; ++n) {
+        const int k = S.nzk[o][n];
+        const double ek = S.nzek[o][n], v2 = S.nzv2[o][n], v = S.nzv[o][n];
+        double pde = 0.0, pdv = 0.0;
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const cxd d = inv_den(ek, g.zi[j]);
+            const double ure = fma(r.re[j], d.re, r.im[j] * d.im);
+            const double uim = fma(r.re[j], d.im, -(r.im[j] * d.re));
+            const double rw2 = 2.0 * g.rw[j];
+            pde = fma(v2 * fma(ure, d.re, -(uim * d.im)), rw2, pde);
+            pdv = fma(2.0 * v * ure, rw2, pdv);
+        }
+        pde = wave_sum(pde);
+        pdv = wave_sum(pdv);
+        if (lane == 0) {
+            S.gred[wid][k] = pde;
+            S.gred[wid][kFitMaxBath + k] = pdv;
+        }
+    }
+    double pdef = 0.0;
+#pragma unroll
+    for (int j = 0; j < JN; ++j) pdef = fma(r.re[j], 2.0 * g.rw[j], pdef);
+    pdef = wave_sum(pdef);
+    if (lane == 0) S.gred[wid][2 * kFitMaxBath] = pdef;
+}
+
 // dfunc (func.f90:47-142) at xt into out[0:nx] (LDS).  Reference order: d/de_k runs over o, then w
 // (one chain through all orbital passes); d/dV(o,k) and d/def_o over w.
 __device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
@@ -344,33 +395,12 @@ __device__ __noinline__ void eval_grad_tree(const FitProb qa, double* out) {
     for (int o = 0; o < q.norb; ++o) {
         for (int base = 0; base < q.nw; base += kGrp * kFitThreads) {
             const FreqGroup g = freq_group(q, base);
-            const GroupResidual r = residual_group(q, S.xt, o, g);
-            const int nn = uniform_i32(S.nnz[o]);
-            for (int n = 0; n < nn; ++n) {
-                const int k = S.nzk[o][n];
-                const double ek = S.nzek[o][n], v2 = S.nzv2[o][n], v = S.nzv[o][n];
-                double pde = 0.0, pdv = 0.0;
-#pragma unroll
-                for (int j = 0; j < kGrp; ++j) {
-                    const cxd d = inv_den(ek, g.zi[j]);
-                    const double ure = fma(r.re[j], d.re, r.im[j] * d.im);
-                    const double uim = fma(r.re[j], d.im, -(r.im[j] * d.re));
-                    const double rw2 = 2.0 * g.rw[j];
-                    pde = fma(v2 * fma(ure, d.re, -(uim * d.im)), rw2, pde);
-                    pdv = fma(2.0 * v * ure, rw2, pdv);
-                }
-                pde = wave_sum(pde);
-                pdv = wave_sum(pdv);
-                if (lane == 0) {
-                    S.gred[wid][k] = pde;
-                    S.gred[wid][kFitMaxBath + k] = pdv;
-                }
+            switch (g.jn) {
+                case 4: grad_group<4>(q, o, g, lane, wid); break;
+                case 3: grad_group<3>(q, o, g, lane, wid); break;
+                case 2: grad_group<2>(q, o, g, lane, wid); break;
+                default: grad_group<1>(q, o, g, lane, wid); break;
             }
-            double pdef = 0.0;
-#pragma unroll
-            for (int j = 0; j < kGrp; ++j) pdef = fma(r.re[j], 2.0 * g.rw[j], pdef);
-            pdef = wave_sum(pdef);
-            if (lane == 0) S.gred[wid][2 * kFitMaxBath] = pdef;
             __syncthreads();
             if (tid < kFitCols) {
                 const int c = tid, k = c < kFitMaxBath ? c : c - kFitMaxBath;
