@@ -201,6 +201,34 @@ def test_green_8site_against_oracle(configs):
     assert len(recs) == c.norb and all(abs(r["bp"][0] + r["bn"][0] - 1) < 1e-12 for r in recs)
 
 
+def test_green_12site_against_oracle(configs):
+    """the 12-site workload (sector (6, 6), D = 853,776, 150-step continued fractions on 1024
+    frequencies): ground state, apply_c, the reference Lanczos recurrence and the continued
+    fraction on the GPU against the oracle's green_diag (OpenMP over the host cores) on the same
+    eigenvector, within 1e-8"""
+    import os as _os
+    c = configs["ns12_plain"]
+    nu, nd = c.sectors[0]
+    om = omega_grid(c.nw, c.beta)
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+        vec = s.eigvec(1, 1)
+    m = omodel(c)
+    b = oracle.Basis(c.nsite, nu, nd)
+    Gr = np.zeros((c.norb, c.nw), complex)
+    nt = min(16, _os.cpu_count() or 1)
+    for (E, p, isec, ilev) in kept:
+        assert (isec, ilev) == (1, 1)
+        for iorb in range(1, c.norb + 1):  # N_up == N_dn: spin up only, factor 1
+            print("oracle green_diag orbital", iorb, flush=True)
+            g, _, _ = oracle.green_diag(m, b, vec, E, p, 1.0, iorb, 1, c.nstep, om, nthreads=nt)
+            Gr[iorb - 1] += g
+    err = np.max(np.abs(G - Gr)) / np.max(np.abs(Gr))
+    print("G rel err", err, flush=True)
+    assert err <= 1e-8
+
+
 def test_green_exact_against_lehmann():
     rng = np.random.default_rng(5)
     norb, nbath, nu, nd = 2, 3, 2, 3
