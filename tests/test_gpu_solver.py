@@ -90,6 +90,30 @@ def test_solve_ground_state_12site_against_arpack(configs):
     assert abs(ev[0] - E0) <= 1e-10 * abs(E0)
 
 
+def test_solve_and_green_14site(configs):
+    """the 14-site workload (sector (7, 7), D = 11,778,624): E0 against SciPy's ARPACK (tol = 0)
+    within 1e-10; the Green's functions on 2048 frequencies: particle and hole weights of every
+    orbital add up to 1 (b+_0 + b-_0, ed_green.F90:174) and Im G(i w_n) < 0 at every frequency"""
+    c = configs["ns14_roofline"]
+    e0 = json.load(open(os.path.join(GOLD, "e0.json"))).get("ns14_roofline")
+    if e0 is None:
+        pytest.skip("no ARPACK golden energy for the 14-site sector")
+    om = omega_grid(c.nw, c.beta)
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        ev = s.sector_eigenvalues(1)
+        G = s.ed_calc_green_ftn(om)
+        recs = s.green_records()
+    E0 = float.fromhex(e0["E0"])
+    print("E0 gpu", repr(ev[0]), "arpack", repr(E0), flush=True)
+    assert abs(ev[0] - E0) <= 1e-10 * abs(E0)
+    assert kept[0][0] == ev[0]
+    assert len(recs) == c.norb
+    for r in recs:
+        assert abs(r["bp"][0] + r["bn"][0] - 1.0) < 1e-12
+    assert np.all(G.imag < 0.0)
+
+
 @pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
 def test_solve_all_sectors_small_model_and_selection(flags):
     """every sector of a 5-site model (dims 1..100): eigenvalues vs dense ED, state selection
