@@ -165,21 +165,27 @@ __device__ __forceinline__ cxd inv_den(double ek, double zi) {
     return {c * inv, -(zi * inv)};
 }
 
-// the nonzero couplings of every orbital at xt, in k order (a thread per orbital); ends with a barrier
+// the nonzero couplings of every orbital at xt, in k order; ends with a barrier.  Wave 0 holds
+// (orbital o, bath k) in lane 16 o + k; a ballot of the nonzero ones gives each its place in its
+// orbital's list (the number of nonzero k' < k).
+static_assert(kMaxOrb * 16 <= 64 && kFitMaxBath <= 16, "one wave holds every (orbital, bath) pair");
 __device__ __forceinline__ void nonzero_couplings(const FitView& q) {
-    const int o = threadIdx.x;
-    if (o < q.norb) {
-        int n = 0;
-        for (int k = 0; k < q.nbath; ++k) {
-            const double v = S.xt[q.nbath + o * q.nbath + k];
-            if (v == 0.0) continue;
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int o = tid >> 4, k = tid & 15;
+        const bool in = o < q.norb && k < q.nbath;
+        const double v = in ? S.xt[q.nbath + o * q.nbath + k] : 0.0;
+        const bool nz = in && v != 0.0;
+        const uint64_t mask = __ballot(nz);
+        const uint64_t seg = mask & (0xffffull << (16 * o));
+        if (nz) {
+            const int n = __popcll(seg & ((1ull << tid) - 1ull));
             S.nzv2[o][n] = v * v;
             S.nzv[o][n] = v;
             S.nzek[o][n] = S.xt[k];
             S.nzk[o][n] = k;
-            ++n;
         }
-        S.nnz[o] = n;
+        if (k == 0 && o < q.norb) S.nnz[o] = __popcll(seg);
     }
     __syncthreads();
 }
