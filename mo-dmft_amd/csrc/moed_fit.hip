@@ -226,6 +226,14 @@ template <int JN>
 __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const double* xt, int o,
                                                         const FreqGroup& g) {
     GroupResidual r;
+    // the target's loads go out first, so that their latency overlaps the sums over the couplings
+    double dre[JN], dim[JN];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+        const size_t io = (size_t)o + (size_t)q.norb * (g.iw[j] >= 0 ? g.iw[j] : 0);
+        dre[j] = q.dev[2 * io];
+        dim[j] = q.dev[2 * io + 1];
+    }
 #pragma unroll
     for (int j = 0; j < kGrp; ++j) r.re[j] = r.im[j] = 0.0;
     const int nn = uniform_i32(S.nnz[o]);
@@ -242,10 +250,8 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
         const bool ok = g.iw[j] >= 0;
-        const size_t io = (size_t)o + (size_t)q.norb * (ok ? g.iw[j] : 0);
-        const double dre = q.dev[2 * io], dim = q.dev[2 * io + 1];
-        r.re[j] = ok ? (r.re[j] + ef) - dre : 0.0;
-        r.im[j] = ok ? r.im[j] - dim : 0.0;
+        r.re[j] = ok ? (r.re[j] + ef) - dre[j] : 0.0;
+        r.im[j] = ok ? r.im[j] - dim[j] : 0.0;
     }
     return r;
 }
