@@ -365,19 +365,22 @@ int get_sector(moed_handle* h, int nu, int nd, Sector* s) {
     return MOED_OK;
 }
 
-HvPlan hv_plan_of(moed_handle* h, const Sector& s) {
-    return hv_plan(s.up->count, s.dn->count, h->hv_threads, h->hv_strip, h->ncu);
+// ldx: row stride of the vectors in doubles (0: nU, the reference layout)
+HvPlan hv_plan_of(moed_handle* h, const Sector& s, int64_t ldx = 0) {
+    return hv_plan(s.up->count, s.dn->count, h->hv_threads, h->hv_strip, h->ncu, ldx > 0 ? ldx : s.up->count);
 }
 
 // number of x.y partials hv_dev writes to dot_part
-int hv_groups(moed_handle* h, const Sector& s) { return hv_plan_of(h, s).grid_b; }
+int hv_groups(moed_handle* h, const Sector& s, int64_t ldx = 0) { return hv_plan_of(h, s, ldx).grid_b; }
 
-// y = H x (x != y) on stream st; optional fused y *= 1/yscale[0] and x.y partials
+// y = H x (x != y) on stream st; optional fused y *= 1/yscale[0] and x.y partials.  ldx: row stride
+// of x and y (0: nU); padding columns of x must hold zeros and are never written in y.
 int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* dot_part, const double* yscale,
-           const int32_t* stop, hipStream_t st) {
+           const int32_t* stop, hipStream_t st, int64_t ldx = 0) {
     HvArgs a{};
     a.x = x;
     a.y = y;
+    a.ldx = ldx > 0 ? ldx : s.up->count;
     a.up = s.up->dev();
     a.dn = s.dn->dev();
     a.ek = h->d_ek.as<double>();
@@ -390,7 +393,7 @@ int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* 
     a.dot_part = dot_part;
     a.yscale = yscale;
     a.stop = stop;
-    HIPCHK(launch_hv(a, hv_plan_of(h, s), h->norb, st));
+    HIPCHK(launch_hv(a, hv_plan_of(h, s, a.ldx), h->norb, st));
     return MOED_OK;
 }
 
@@ -444,8 +447,15 @@ const double kBreakdown = (double)1.e-12f;  // ed_lanczos.F90:62, a REAL*4 liter
 int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep, int* nsc, double* a, double* b) {
     if (nstep < 1) return fail(h, MOED_EARG, "nstep must be >= 1");
     hipStream_t st = h->stream;
-    const int64_t D = s.D;
-    const int ngrp = hv_groups(h, s);
+    // rows of odd length run padded to an even stride, so that the H v strip pass reads 16-byte
+    // pairs; the padding columns hold zeros throughout (H v never writes them and the three-term
+    // updates of zeros stay zero), so every sum is over the sector's own elements.  With three or
+    // more orbitals the two-column strip kernel holds too many registers (153 VGPRs, 3 waves per
+    // SIMD) and is slower than the one-column kernel on the unpadded rows, so those stay as they are.
+    const int64_t nU = s.up->count, nD = s.dn->count;
+    const int64_t ldx = (nU % 2 == 1 && h->hv_strip != 64 && h->norb <= 2) ? nU + 1 : nU;
+    const int64_t D = nD * ldx;
+    const int ngrp = hv_groups(h, s, ldx);
     DevBuf& V = h->ws[WS_V];
     DevBuf& W = h->ws[WS_W];
     DevBuf& part = h->ws[WS_PART];
@@ -463,17 +473,23 @@ int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep
     HIPCHK(launch_fill(0.0, 2 * (nstep + 1), scal.as<double>(), st));
     const int32_t init[2] = {0, nstep};
     HIPCHK(hipMemcpyAsync(sc.stop, init, sizeof(init), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(u[0], u0, D * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (ldx != nU) {
+        HIPCHK(launch_fill(0.0, 3 * D, V.as<double>(), st));
+        HIPCHK(launch_fill(0.0, D, w, st));
+        HIPCHK(launch_pad_rows(u0, nU, nD, u[0], ldx, st));
+    } else {
+        HIPCHK(hipMemcpyAsync(u[0], u0, D * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
     // B[0] = |u_0|^2 ; w = H u_0 ; A[0] = <u_0, w> / B[0]
     HIPCHK(launch_dot_partial(u[0], u[0], D, p, st));
     HIPCHK(launch_lz_finish_b(0, p, kRedBlocks, sc, st));
-    int rc = hv_dev(h, s, u[0], w, p, nullptr, sc.stop, st);
+    int rc = hv_dev(h, s, u[0], w, p, nullptr, sc.stop, st, ldx);
     if (rc) return rc;
     HIPCHK(launch_lz_finish_a(0, p, ngrp, sc, kBreakdown, st));
     for (int j = 1; j <= nstep; ++j) {
         HIPCHK(launch_lz_update(j, w, u[(j - 1) % 3], u[(j + 1) % 3], u[j % 3], D, sc, p, st));
         HIPCHK(launch_lz_finish_b(j, p, kRedBlocks, sc, st));
-        rc = hv_dev(h, s, u[j % 3], w, p, nullptr, sc.stop, st);
+        rc = hv_dev(h, s, u[j % 3], w, p, nullptr, sc.stop, st, ldx);
         if (rc) return rc;
         HIPCHK(launch_lz_finish_a(j, p, ngrp, sc, kBreakdown, st));
     }

@@ -62,8 +62,8 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     const int id = blockIdx.x;
     const int neg = nU + 1;
     const int tid = threadIdx.x;
-    const double* __restrict__ xr = P.x + (size_t)id * nU;
-    double* __restrict__ yr = P.y + (size_t)id * nU;
+    const double* __restrict__ xr = P.x + (size_t)id * P.ldx;
+    double* __restrict__ yr = P.y + (size_t)id * P.ldx;
     const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(BYTES ? P.up.rec8 : P.up.rec);
     const int rstride = 4 * P.up.nq;
     uint32_t wn[NW];
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
 
 // ------------------------------------------------------------------ strip pass
 
-// CPL consecutive columns of one row as one load (CPL = 2 needs an even nU: 16-byte aligned)
+// CPL consecutive columns of one row as one load (CPL = 2 needs an even row stride: 16-byte aligned)
 template <int CPL>
 struct Cols {
     double v[CPL];
@@ -205,8 +205,11 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
     if (item < nstrip * bps) {
         const int strip = item / bps, rb = item % bps;
         const int c00 = strip * W + lane * CPL;
-        const bool ok = c00 < nU;          // with CPL = 2 and nU even, both columns are valid
-        const int iu = ok ? c00 : nU - CPL;  // idle lanes read valid columns and store nothing
+        const bool ok = c00 < nU;
+        // with CPL = 2 and an odd nU (rows padded to an even stride) the last pair ends in the
+        // padding column: it is read (zero) but never stored; idle lanes read an aligned valid pair
+        const bool ok1 = CPL == 1 || c00 + 1 < nU;
+        const int iu = ok ? c00 : ((nU - CPL) & ~(CPL - 1));
         const int r1 = min(nD, (rb + 1) * P.rows_b);
         // up half of the spin-flip / pair-exchange terms: move j -> i of each column's up string
         int32_t mu[NP > 1 ? NP : 1][CPL];
@@ -215,10 +218,10 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                    mu[p][c] = (p / NORB != p % NORB) ? P.up.imp[(size_t)p * nU + iu + c] : -1;
+                    mu[p][c] = (p / NORB != p % NORB && (c == 0 || ok1)) ? P.up.imp[(size_t)p * nU + iu + c] : -1;
         }
         for (int id = rb * P.rows_b + wave; id < r1; id += NW) {
-            const size_t ro = (size_t)id * nU;
+            const size_t ro = (size_t)id * P.ldx;
             // ---- scalar loads: the row's first eight down moves and its down impurity moves
             const int nh = nhopc[id];
             const size_t m0 = (size_t)id * mh;
@@ -245,7 +248,7 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
             Cols<CPL> xv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                if (q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * nU + iu);
+                if (q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * P.ldx + iu);
             double gs[NP > 1 ? NP : 1][CPL], gp[NP > 1 ? NP : 1][CPL];
             if constexpr (NORB > 1) {
 #pragma unroll
@@ -254,8 +257,8 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
                         const int uu = mu[p][c] >= 0 ? (mu[p][c] & 0xffff) : iu + c;
-                        if (msf[p] >= 0) gs[p][c] = X[(size_t)(msf[p] & 0xffff) * nU + uu];
-                        if (mpe[p] >= 0) gp[p][c] = X[(size_t)(mpe[p] & 0xffff) * nU + uu];
+                        if (msf[p] >= 0) gs[p][c] = X[(size_t)(msf[p] & 0xffff) * P.ldx + uu];
+                        if (mpe[p] >= 0) gp[p][c] = X[(size_t)(mpe[p] & 0xffff) * P.ldx + uu];
                     }
                 }
             }
@@ -274,7 +277,7 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 }
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                    if (k0 + q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * nU + iu);
+                    if (k0 + q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * P.ldx + iu);
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (k0 + q < nh) {
@@ -303,15 +306,11 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 for (int c = 0; c < CPL; ++c) acc.v[c] *= inv;
             }
             if (ok) {
-                if constexpr (CPL == 2) {
-                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
-                    __builtin_nontemporal_store(acc.v[1], Y + ro + iu + 1);
-                } else {
-                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
-                }
+                __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
+                if (CPL == 2 && ok1) __builtin_nontemporal_store(acc.v[CPL - 1], Y + ro + iu + 1);
                 if (P.dot_part) {
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) dsum = fma(xo.v[c], acc.v[c], dsum);
+                    dsum = fma(xo.v[0], acc.v[0], dsum);
+                    if (CPL == 2 && ok1) dsum = fma(xo.v[CPL - 1], acc.v[CPL - 1], dsum);
                 }
             }
         }
@@ -335,13 +334,13 @@ int record_dwords(int nslot) {
 
 // threads_a: row pass threads per block (0 = 1024); strip: strip width 64 or 128 (0 = 128);
 // ncu: compute units of the device
-HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu) {
+HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
     HvPlan p{};
     p.threads_a = threads_a > 0 ? threads_a : 1024;
     p.grid_a = nD;
     (void)ncu;
     if (strip <= 0) strip = 128;
-    p.cpl = (strip >= 128 && (nU % 2) == 0) ? 2 : 1;
+    p.cpl = (strip >= 128 && (ldx % 2) == 0 && (nU % 2 == 0 || nU >= 3)) ? 2 : 1;
     // rows per work item: 8 per wave with 128-column strips, 32 with 64-column strips (items of
     // 1-2 K elements; measured best at the 14-site (7, 7) and 15-site (7, 8) sectors)
     p.rows_b = (p.cpl == 2 ? 2 : 8) * (kStripThreads / 64);
@@ -409,11 +408,11 @@ static hipError_t hv_launch_norb(const HvArgs& a0, const HvPlan& pl, hipStream_t
     hipError_t e = rows_launch<NORB>(a, pl, s);
     if (e != hipSuccess) return e;
     const bool al16 = ((reinterpret_cast<uintptr_t>(a.x) | reinterpret_cast<uintptr_t>(a.y)) & 15) == 0;
-    if (pl.cpl == 2 && (a.up.count % 2) == 0 && !al16) {
+    if (pl.cpl == 2 && !al16) {
         // 16-byte pairs need 16-byte aligned x and y: a caller's misaligned vectors take the
         // 64-column strips; the x.y partials are sized for the plan's strips, so none are asked for here
         if (a.dot_part) return hipErrorInvalidValue;
-        const HvPlan p1 = hv_plan(a.up.count, a.dn.count, pl.threads_a, 64, 0);
+        const HvPlan p1 = hv_plan(a.up.count, a.dn.count, pl.threads_a, 64, 0, a.ldx);
         a.rows_b = p1.rows_b;
         a.items_per_xcd = p1.items_per_xcd;
         hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(p1.grid_b), dim3(kStripThreads), 0, s, a);

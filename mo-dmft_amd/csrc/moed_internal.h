@@ -67,6 +67,7 @@ struct HvArgs {
     const int32_t* stop;      // optional device flag: both passes are no-ops when *stop != 0
     int32_t rows_b;           // strip pass: rows per block
     int32_t items_per_xcd;    // strip pass: (strip, row block) items per XCD
+    int64_t ldx;              // row stride of x and y in doubles (nU, or nU + 1 for padded odd rows)
 };
 
 constexpr int kStripThreads = 256;
@@ -77,7 +78,8 @@ struct HvPlan {
     int cpl;                            // strip pass: columns per lane (strip width 64 * cpl)
     int rows_b, items_per_xcd, grid_b;  // grid_b = number of x.y partials
 };
-HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu);
+// ldx: row stride of the vectors (nU; 128-column strips need an even ldx, and x, y 16-byte aligned)
+HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx);
 hipError_t launch_hv(const HvArgs& a, const HvPlan& plan, int norb, hipStream_t s);
 // record dwords the row pass reads for nslot move slots (3, 5, 7, 9, 12, 16, 24 or 32)
 int record_dwords(int nslot);
@@ -168,6 +170,9 @@ struct ApplyCArgs {
 hipError_t launch_apply_c(const ApplyCArgs& a, hipStream_t s);
 hipError_t launch_fill_start(uint64_t seed, int64_t n, double* x, hipStream_t s);
 hipError_t launch_fill(double v, int64_t n, double* x, hipStream_t s);
+// dst[id * ldx + iu] = src[id * nU + iu]: rows of nU doubles copied to a stride of ldx (the padding
+// columns of dst are left as they are)
+hipError_t launch_pad_rows(const double* src, int64_t nU, int64_t nD, double* dst, int64_t ldx, hipStream_t s);
 hipError_t launch_scale_copy(const double* x, double alpha, int64_t n, double* y, hipStream_t s);
 
 // deterministic reductions: partial sums per block then one final block in fixed order

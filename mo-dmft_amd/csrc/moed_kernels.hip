@@ -73,6 +73,21 @@ hipError_t launch_fill(double v, int64_t n, double* x, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void pad_rows_kernel(const double* __restrict__ src, int64_t nU, int64_t n, double* __restrict__ dst,
+                                int64_t ldx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t id = i / nU;
+    dst[id * ldx + (i - id * nU)] = src[i];
+}
+
+hipError_t launch_pad_rows(const double* src, int64_t nU, int64_t nD, double* dst, int64_t ldx, hipStream_t s) {
+    const int64_t n = nU * nD;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, nU, n, dst, ldx);
+    return hipGetLastError();
+}
+
 __global__ void scale_copy_kernel(const double* x, double alpha, int64_t n, double* y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = alpha * x[i];

@@ -205,26 +205,41 @@ def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
     """moed_hv_device with x and y starting at either 8-byte phase (odd nU at 15 sites, where every
     C(15, k) is odd, and even nU at 12 sites, whose 16-byte row and strip loads need 16-byte aligned
     vectors and otherwise fall back to 8-byte loads): y is bit-identical to the host-buffer product
-    and to the 64-column (8-byte load) strips in every case, and nothing is written outside y"""
-    import torch
+    and to the 64-column (8-byte load) strips in every case, and nothing is written outside y.  The
+    device buffers come from the HIP runtime the library uses."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    hip.hipFree.argtypes = [ct.c_void_p]
+    H2D, D2H = 1, 2
     c = configs_by_name()[name]
     nu, nd = sector
     with EDSolver.from_config(c) as s:
         D = s.dim(nu, nd)
         x = start_vector(nu * 31 + nd, D)
         ref = s.multiply_H(nu, nd, x)
-        for xoff in (0, 1):
-            for yoff in (0, 1):
-                print("case", name, sector, xoff, yoff, flush=True)
-                xb = torch.zeros(D + 2, dtype=torch.float64, device="cuda")
-                yb = torch.zeros(D + 2, dtype=torch.float64, device="cuda")
-                xb[xoff:xoff + D] = torch.from_numpy(x)
-                torch.cuda.synchronize()
-                s.multiply_H_device(nu, nd, xb.data_ptr() + 8 * xoff, yb.data_ptr() + 8 * yoff)
-                torch.cuda.synchronize()
-                y = yb.cpu().numpy()
-                assert np.array_equal(y[yoff:yoff + D], ref)
-                assert y[:yoff].sum() == 0.0 and y[yoff + D:].sum() == 0.0  # nothing written outside y
+        nbytes = 8 * (D + 2)
+        bufs = [ct.c_void_p(), ct.c_void_p()]
+        for b in bufs:
+            assert hip.hipMalloc(ct.byref(b), nbytes) == 0
+        try:
+            for xoff in (0, 1):
+                for yoff in (0, 1):
+                    print("case", name, sector, xoff, yoff, flush=True)
+                    xh = np.zeros(D + 2)
+                    xh[xoff:xoff + D] = x
+                    assert hip.hipMemcpy(bufs[0], xh.ctypes.data, nbytes, H2D) == 0
+                    assert hip.hipMemcpy(bufs[1], np.zeros(D + 2).ctypes.data, nbytes, H2D) == 0
+                    s.multiply_H_device(nu, nd, bufs[0].value + 8 * xoff, bufs[1].value + 8 * yoff)
+                    assert hip.hipDeviceSynchronize() == 0
+                    y = np.empty(D + 2)
+                    assert hip.hipMemcpy(y.ctypes.data, bufs[1], nbytes, D2H) == 0
+                    assert np.array_equal(y[yoff:yoff + D], ref)
+                    assert y[:yoff].sum() == 0.0 and y[yoff + D:].sum() == 0.0  # nothing written outside y
+        finally:
+            for b in bufs:
+                hip.hipFree(b)
         s.set_hv_shape(0, 64)
         assert np.array_equal(s.multiply_H(nu, nd, x), ref)
 

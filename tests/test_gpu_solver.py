@@ -253,6 +253,36 @@ def test_green_12site_against_oracle(configs):
     assert err <= 1e-8
 
 
+@pytest.mark.parametrize("name, sector", [("ns14_roofline", (3, 3)), ("ns14_roofline", (3, 2)),
+                                          ("ns15_fit", (2, 2)), ("ns15_fit", (3, 2))])
+def test_green_odd_rows_against_oracle(configs, name, sector):
+    """chains in sectors whose up-string count is odd: at 14 sites (two orbitals; the targets have
+    C(14, 4) = 1001 and C(14, 2) = 91 up strings) they run on rows padded to an even stride, at 15
+    sites (three orbitals, every C(15, k) odd) unpadded; G equals the oracle's green_diag on the same
+    eigenvector within 1e-8, both spins when N_up != N_dn"""
+    import os as _os
+    c = configs[name]
+    nu, nd = sector
+    om = omega_grid(256, c.beta)
+    with EDSolver.from_config(c, sectors=[sector], nev=1, nstep=60, nw=len(om)) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+        vec = s.eigvec(1, 1)
+    m = omodel(c)
+    b = oracle.Basis(c.nsite, nu, nd)
+    Gr = np.zeros((c.norb, len(om)), complex)
+    nt = min(16, _os.cpu_count() or 1)
+    for (E, p, isec, ilev) in kept:
+        spins, factor = ((1,), 1.0) if nu == nd else ((1, 2), 0.5)
+        for ispin in spins:
+            for iorb in range(1, c.norb + 1):
+                g, _, _ = oracle.green_diag(m, b, vec, E, p, factor, iorb, ispin, 60, om, nthreads=nt)
+                Gr[iorb - 1] += g
+    err = np.max(np.abs(G - Gr)) / np.max(np.abs(Gr))
+    print("G rel err", sector, err, flush=True)
+    assert err <= 1e-8
+
+
 def test_green_exact_against_lehmann():
     rng = np.random.default_rng(5)
     norb, nbath, nu, nd = 2, 3, 2, 3
