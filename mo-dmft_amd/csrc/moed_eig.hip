@@ -212,9 +212,9 @@ hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b,
 }
 
 // part[i * ni + item] = <V_i, w> over the item for the VG basis vectors of group blockIdx.y
-// (i0 <= i < k1)
+// (i0 <= i < k1); V[i] is the address of basis vector i
 constexpr int kEigVG = 8;
-__global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, const double* V, const double* w,
+__global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, const double* const* V, const double* w,
                                                                    int i0, int k1, int pass) {
     __shared__ double red[kEigVG][kEigThreads / 64];
     const int item = blockIdx.x;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_kernel(EigArgs e, co
         const int i = g0 + q;
         double s = 0.0;
         if (i < k1) {
-            const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+            const double* __restrict__ u = V[i] + base;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int r = threadIdx.x + j * kEigThreads;
@@ -278,8 +278,9 @@ __global__ void __launch_bounds__(kEigThreads) eig_reduce_h_kernel(EigArgs e, in
 }
 
 // dst = src - sum_{i0 <= i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item
-__global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, const double* V, const double* src,
-                                                                  double* dst, int i0, int k1, int pass) {
+__global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, const double* const* V,
+                                                                  const double* src, double* dst, int i0, int k1,
+                                                                  int pass) {
     extern __shared__ double sc[];
     __shared__ double red[kEigThreads / 64];
     const int item = blockIdx.x;
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
 #pragma unroll
             for (int q = 0; q < 8; ++q)
 #pragma unroll
-                for (int j = 0; j < R; ++j) u[q][j] = V[(size_t)(i + q) * e.dtot + base + threadIdx.x + j * kEigThreads];
+                for (int j = 0; j < R; ++j) u[q][j] = V[i + q][base + threadIdx.x + j * kEigThreads];
 #pragma unroll
             for (int q = 0; q < 8; ++q)
 #pragma unroll
@@ -312,11 +313,11 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
         }
         for (; i < k1; ++i)
 #pragma unroll
-            for (int j = 0; j < R; ++j) acc[j] = fma(-sc[i], V[(size_t)i * e.dtot + base + threadIdx.x + j * kEigThreads], acc[j]);
+            for (int j = 0; j < R; ++j) acc[j] = fma(-sc[i], V[i][base + threadIdx.x + j * kEigThreads], acc[j]);
     } else {
         for (int i = i0; i < k1; ++i) {
             const double ci = sc[i];
-            const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+            const double* __restrict__ u = V[i] + base;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int r = threadIdx.x + j * kEigThreads;
@@ -436,12 +437,12 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
 // part, written to slot k + 1; pass 1 (only where pass 0's overlap estimates ask for it): slot
 // k + 1 against the whole basis V_0..V_k, in place; pass 2 (only where pass 1's DGKS test asks
 // for it): pass 1 again.
-hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
-                           double tiny, hipStream_t s) {
+hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1, const double* w, int k, int pass,
+                           double eta2, double tiny, hipStream_t s) {
     const int k1 = k + 1;
     const int i0 = pass == 0 ? (k > 0 ? k - 1 : 0) : 0;
-    const double* src = pass == 0 ? w : V + (size_t)k1 * e.dtot;
-    double* dst = const_cast<double*>(V) + (size_t)k1 * e.dtot;
+    const double* src = pass == 0 ? w : vk1;
+    double* dst = vk1;
     const int nv = k1 - i0;
     const int ng = (nv + kEigVG - 1) / kEigVG;
     hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, i0, k1, pass);
@@ -455,8 +456,9 @@ hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, i
 // ------------------------------------------------------------------ Ritz vectors
 
 // y = sum_{i < m_p} S_i V_i over the item, written to out[p] + j*dim_p; part[j*ni + item] = |y|^2
-__global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const double* V, const double* S, int lds,
-                                                               int nev, const int32_t* nf, double* const* out) {
+__global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const double* const* V, const double* S,
+                                                               int lds, int nev, const int32_t* nf,
+                                                               double* const* out) {
     extern __shared__ double ss[];
     __shared__ double red[kEigThreads / 64];
     const int item = blockIdx.x, j = blockIdx.y;
@@ -474,7 +476,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const 
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
     for (int i = 0; i < m; ++i) {
         const double si = ss[i];
-        const double* __restrict__ u = V + (size_t)i * e.dtot + base;
+        const double* __restrict__ u = V[i] + base;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int t = threadIdx.x + r * kEigThreads;
@@ -514,7 +516,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_ritz_scale_kernel(EigArgs e, 
     for (int t = threadIdx.x; t < len; t += kEigThreads) y[t] = y[t] * f;
 }
 
-hipError_t launch_eig_ritz(const EigArgs& e, const double* V, const double* S, int lds, int nev,
+hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s) {
     hipLaunchKernelGGL(eig_ritz_kernel, dim3(e.ni, nev), dim3(kEigThreads), (size_t)lds * sizeof(double), s, e, V, S,
                        lds, nev, nf, out);

@@ -113,6 +113,7 @@ struct moed_handle {
     uint32_t flags = 0;
     double gs_tol = 1e-12;
     int gs_maxvec = 400;
+    int eig_chunk = 0;  // Krylov basis vectors per allocation (0: what fits in kBasisBytes)
     std::vector<double> ek, vk, tij;
     bool has_tij = false;
     std::vector<int> nelec, nup;
@@ -133,6 +134,7 @@ struct moed_handle {
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
+    std::vector<std::unique_ptr<DevBuf>> basis;  // its Krylov basis, in chunks of whole vectors
 };
 
 namespace {
@@ -554,7 +556,15 @@ int eig_dense(moed_handle* h, const Sector& s, int nev, SectorEig& E) {
 // residual |beta_m z_{m,i}| <= tol |theta_i| (tested on the host after steps m < 20 and every
 // fifth step after, with one synchronisation for all problems).  Ritz vectors V S on the device.
 // The vectors are packed end to end; every launch serves all problems (moed_eig.hip).
-enum { EG_V = 0, EG_W, EG_I32, EG_I64, EG_DBL, EG_DESC, EG_OUT, EG_N };
+enum { EG_VT = 0, EG_W, EG_I32, EG_I64, EG_DBL, EG_DESC, EG_OUT, EG_N };
+
+// The Krylov basis is allocated as the iteration reaches it, in chunks of whole vectors
+// (eig_chunk vectors; default: what fits in kBasisBytes), and the handle keeps the chunks for
+// later calls.  Its cap (gs_maxvec + 1 vectors: 401 vectors of 331 MB at the 15-site sector) is
+// far more than a solve uses, and device allocations cost ~30 ms/GiB once the memory has been used
+// before (the driver clears it), so allocating the cap up front made the first solve 2-4 s instead
+// of 0.3 s.  Kernels address the vectors through a device table of their addresses.
+constexpr size_t kBasisBytes = (size_t)4 << 30;
 
 int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out) {
     hipStream_t st = h->stream;
@@ -604,7 +614,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     // device memory (handle-owned, grown as needed)
     DevBuf* g = h->eng;
     const size_t npart = std::max<size_t>((size_t)ldh * ni, (size_t)nev * ni);
-    HIPCHK(g[EG_V].ensure((size_t)(M + 1) * dtot * sizeof(double)));
+    HIPCHK(g[EG_VT].ensure((size_t)(M + 1) * sizeof(double*)));
     HIPCHK(g[EG_W].ensure((size_t)dtot * sizeof(double)));
     const size_t n32 = 2 * (size_t)ni + 9 * (size_t)P + 2 * (size_t)nr;
     HIPCHK(g[EG_I32].ensure(n32 * sizeof(int32_t)));
@@ -648,7 +658,6 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     EigArgs e{};
     e.P = P;
     e.ni = ni;
-    e.dtot = dtot;
     e.ldh = ldh;
     e.item_prob = d_item_prob;
     e.item_start = d_item_start;
@@ -672,11 +681,40 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     e.full = d_full;
     e.force = d_force;
 
-    double* V = g[EG_V].as<double>();
+    // basis vectors: those the handle's chunks already hold, then new chunks as steps need them
+    const size_t vbytes = (size_t)dtot * sizeof(double);
+    const int64_t per = h->eig_chunk > 0 ? h->eig_chunk : std::max<int64_t>(1, (int64_t)(kBasisBytes / vbytes));
+    double** vt = g[EG_VT].as<double*>();
+    std::vector<double*> vp;
+    vp.reserve((size_t)M + 1);  // the table's host copy never moves (queued copies read it)
+    size_t vp_dev = 0;          // entries already queued to the device table
+    for (auto& c : h->basis)
+        for (size_t o = 0; o + vbytes <= c->bytes && (int64_t)vp.size() < M + 1; o += vbytes)
+            vp.push_back(reinterpret_cast<double*>(static_cast<char*>(c->p) + o));
+    auto reserve = [&](int64_t need) -> int {
+        while ((int64_t)vp.size() < need) {
+            const int64_t nv = std::min<int64_t>(per, M + 1 - (int64_t)vp.size());
+            auto c = std::make_unique<DevBuf>();
+            if (c->ensure((size_t)nv * vbytes) != hipSuccess)
+                return fail(h, MOED_ENOMEM, "eigensolver: allocating " + std::to_string(nv) + " Krylov vectors of " +
+                                                std::to_string(vbytes) + " bytes failed");
+            for (int64_t j = 0; j < nv; ++j)
+                vp.push_back(reinterpret_cast<double*>(static_cast<char*>(c->p) + (size_t)j * vbytes));
+            h->basis.push_back(std::move(c));
+        }
+        if (vp.size() > vp_dev) {
+            HIPCHK(hipMemcpyAsync(vt + vp_dev, vp.data() + vp_dev, (vp.size() - vp_dev) * sizeof(double*),
+                                  hipMemcpyHostToDevice, st));
+            vp_dev = vp.size();
+        }
+        return MOED_OK;
+    };
+    if (int rc = reserve(2)) return rc;
     double* W = g[EG_W].as<double>();
     for (int p = 0; p < P; ++p)
-        HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p], V + off[p], st));
-    HIPCHK(launch_eig_segdot(e, V, V, d_nrm, ldh, 1, st));  // nrm[p][0] = |u_0|
+        HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p], vp[0] + off[p],
+                                 st));
+    HIPCHK(launch_eig_segdot(e, vp[0], vp[0], d_nrm, ldh, 1, st));  // nrm[p][0] = |u_0|
 
     HvBatchArgs hb{};
     hb.prob = g[EG_DESC].as<BatchProbDev>();
@@ -696,7 +734,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     std::vector<double> al_h((size_t)P * ldh), be_h((size_t)P * ldh);
     int active = P;
     for (int k = 0; k < M && active > 0; ++k) {
-        const double* Vk = V + (size_t)k * dtot;
+        if (int rc = reserve(k + 2)) return rc;  // step k writes vector k + 1
+        const double* Vk = vp[k];
         hb.x = Vk;
         hb.scale = d_nrm + k;
         HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
@@ -705,9 +744,9 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
             int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
             if (rc) return rc;
         }
-        HIPCHK(launch_eig_pass(e, V, W, k, 0, 0.5, 1e-14, st));  // three-term part
-        HIPCHK(launch_eig_pass(e, V, W, k, 1, 0.5, 1e-14, st));  // whole basis
-        HIPCHK(launch_eig_pass(e, V, W, k, 2, 0.5, 1e-14, st));  // again where DGKS asks
+        HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 0, 0.5, 1e-14, st));  // three-term part
+        HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 1, 0.5, 1e-14, st));  // whole basis
+        HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 2, 0.5, 1e-14, st));  // again where DGKS asks
         const int m = k + 1;
         if (m < 20 || m % 5 == 0 || m == M) {
             HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -796,7 +835,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     HIPCHK(hipMemcpyAsync(d_S, S_h.data(), S_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_nf, nf.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(g[EG_OUT].p, outp.data(), P * sizeof(double*), hipMemcpyHostToDevice, st));
-    HIPCHK(launch_eig_ritz(e, V, d_S, ldh, nev, d_nf, g[EG_OUT].as<double*>(), d_n2, st));
+    HIPCHK(launch_eig_ritz(e, vt, d_S, ldh, nev, d_nf, g[EG_OUT].as<double*>(), d_n2, st));
     HIPCHK(hipStreamSynchronize(st));
     return MOED_OK;
 }
@@ -1011,6 +1050,7 @@ int moed_destroy(moed_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->tables.clear();
     h->eig.clear();
+    h->basis.clear();
     for (auto& w : h->ws) w.release();
     h->d_ek.release();
     h->d_coef.release();
@@ -1682,6 +1722,16 @@ int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip) {
     if (strip != 0 && strip != 64 && strip != 128) return fail(h, MOED_EARG, "strip must be 0, 64 or 128");
     h->hv_threads = threads;
     h->hv_strip = strip;
+    return MOED_OK;
+}
+
+int moed_set_eig_chunk(moed_handle* h, int32_t vectors) {
+    if (!h) return MOED_EARG;
+    if (vectors < 0) return fail(h, MOED_EARG, "vectors must be 0 (default) or positive");
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    h->eig_chunk = vectors;
+    h->basis.clear();
     return MOED_OK;
 }
 

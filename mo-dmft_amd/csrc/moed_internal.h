@@ -118,7 +118,6 @@ hipError_t launch_hv_batch(const HvBatchArgs& a, int nitems, int max_nU, int nor
 // vector items and per-problem bookkeeping of the engine
 struct EigArgs {
     int32_t P, ni;
-    int64_t dtot;              // packed length (stride between basis vectors)
     int32_t ldh;               // row length of the per-problem scalar tables (>= M + 2)
     const int32_t* item_prob;  // [ni]
     const int64_t* item_start; // [ni] packed index of the item's first element
@@ -145,16 +144,17 @@ struct EigArgs {
 // part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
                              hipStream_t s);
-// one Gram-Schmidt pass at step k (basis V_0..V_k, packed with stride dtot), c_i = <V_i, src> / |V_i|^2:
+// one Gram-Schmidt pass at step k (V: device table of the basis vectors' addresses, each vector
+// packed like w; vk1: the address of vector k + 1), c_i = <V_i, src> / |V_i|^2:
 //   pass 0: V_{k+1} = w - sum_{i = k-1, k} c_i V_i (the three-term part); |V_{k+1}|^2 is the DGKS
 //           reference; the overlap estimates decide whether the step needs pass 1 (partial
 //           reorthogonalisation) — if not, the step's alpha, beta and stop test
 //   pass 1: (problems flagged) V_{k+1} -= sum_{i <= k} c_i V_i in place; DGKS test against the reference
 //   pass 2: (problems flagged) pass 1 again; after pass 1 or 2 the step's alpha, beta and stop test
-hipError_t launch_eig_pass(const EigArgs& e, const double* V, const double* w, int k, int pass, double eta2,
-                           double tiny, hipStream_t s);
+hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1, const double* w, int k, int pass,
+                           double eta2, double tiny, hipStream_t s);
 // Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
-hipError_t launch_eig_ritz(const EigArgs& e, const double* V, const double* S, int lds, int nev,
+hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s);
 
 // apply_c: per target string, the source rank and sign of the elementary operator

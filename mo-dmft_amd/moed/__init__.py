@@ -56,7 +56,7 @@ EXPORTS = [
     "moed_hv", "moed_hv_device", "moed_apply_c", "moed_lanczos", "moed_solve", "moed_sector_eigenvalues",
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
-    "moed_fill_start_device", "moed_set_hv_shape", "moed_stream", "moed_band_tb", "moed_delta_new",
+    "moed_fill_start_device", "moed_set_hv_shape", "moed_set_eig_chunk", "moed_stream", "moed_band_tb", "moed_delta_new",
     "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
     "moed_export_eigvec", "moed_import_eigvec", "moed_export_green", "moed_import_green",
 ]
@@ -99,6 +99,7 @@ def lib():
         L.moed_omega_grid.argtypes = [ct.c_int32, ct.c_double, _dp]
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
+        L.moed_set_eig_chunk.argtypes = [ct.c_void_p, ct.c_int32]
         L.moed_set_fit_launch.argtypes = [ct.c_void_p, ct.c_int32]
         L.moed_dos_diag.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_double, ct.c_int32, ct.c_int32,
                                     _dp, _dp, _dp, _dp, _dp]
@@ -424,6 +425,9 @@ class EDSolver:
 
     def set_hv_shape(self, threads=0, strip=0):
         self._check(lib().moed_set_hv_shape(self.h, threads, strip))
+
+    def set_eig_chunk(self, vectors=0):
+        self._check(lib().moed_set_eig_chunk(self.h, vectors))
 
     def set_fit_launch(self, iterations=0):
         """CG iterations per fit and kernel launch inside minimization (0: default)"""

@@ -203,6 +203,64 @@ def test_batched_solve_and_green_equal_sector_by_sector():
     assert np.allclose(e0[4 * isec:4 * isec + 4], w[:4], rtol=0, atol=1e-10 * abs(w[0]))
 
 
+def _solve_green(s, om, nsec):
+    kept = s.ed_solve()
+    evs = np.concatenate([s.sector_eigenvalues(i + 1) for i in range(nsec)])
+    return kept, evs, s.eigvec(1, 1), s.ed_calc_green_ftn(om)
+
+
+def _same(a, b):
+    assert a[0] == b[0]
+    for u, v in zip(a[1:], b[1:]):
+        assert np.array_equal(u, v)
+
+
+def test_krylov_basis_chunks_12site(configs):
+    """the eigensolver's Krylov basis allocated one and three vectors at a time (kernels address the
+    vectors through a table) gives the same eigenvalues, eigenvector and G(iw), bit for bit, as the
+    default chunk; so does changing the chunk size between two solves on one handle"""
+    c = configs["ns12_plain"]
+    om = omega_grid(c.nw, c.beta)
+    with EDSolver.from_config(c) as s:
+        ref = _solve_green(s, om, 1)
+        s.set_eig_chunk(1)
+        again = _solve_green(s, om, 1)
+    with EDSolver.from_config(c) as s:
+        s.set_eig_chunk(3)
+        chunked = _solve_green(s, om, 1)
+        again3 = _solve_green(s, om, 1)  # the kept chunks serve the second solve
+    print("12-site E0", repr(ref[1][0]), flush=True)
+    _same(ref, again)
+    _same(ref, chunked)
+    _same(ref, again3)
+    with EDSolver.from_config(c) as s:
+        with pytest.raises(moed.MoedError):
+            s.set_eig_chunk(-1)
+
+
+@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
+def test_krylov_basis_chunks_all_sectors(flags):
+    """7-site model, all 64 sectors, sector by sector and in one multi-problem run (sectors of
+    different sizes reuse one handle's chunks): a basis allocated two vectors at a time gives the
+    default chunk's results bit for bit"""
+    rng = np.random.default_rng(78)
+    norb, nbath = 2, 5
+    ns = norb + nbath
+    ek = rng.uniform(-2, 2, ns)
+    vk = np.zeros((norb, nbath))
+    for b in range(nbath):
+        vk[b % norb, b] = rng.uniform(0.2, 0.9)
+    sectors = [(a, b) for a in range(ns + 1) for b in range(ns + 1)]
+    om = moed.omega_grid(256, 20.0)
+    out = []
+    for chunk in (0, 2):
+        with EDSolver(norb, nbath, 3.0, 0.5, ek, vk, beta=20.0, nev=4, nstep=40, sectors=sectors,
+                      flags=flags) as s:
+            s.set_eig_chunk(chunk)
+            out.append(_solve_green(s, om, len(sectors)))
+    _same(out[0], out[1])
+
+
 def test_green_8site_against_oracle(configs):
     c = configs["ns8_plain"]
     nu, nd = c.sectors[0]
