@@ -43,6 +43,28 @@ def test_host_only_entry_points():
     assert lib.moed_omega_grid(-1, 1.0, None) == -1
 
 
+def test_create_rejects_bad_sizes_before_any_device_call():
+    """moed_create validates the configuration before it touches the device: more than 4 orbitals,
+    more than 15 sites, fewer sites than orbitals and negative counts are MOED_EARG"""
+    import numpy as np
+    import moed
+    for norb, nbath, kw in [(5, 2, {}), (2, 14, {}), (3, -1, {}), (2, 2, {"nstep": -1}), (2, 2, {"nev": -1})]:
+        ns = max(norb + nbath, 1)
+        with pytest.raises(moed.MoedError) as ei:
+            moed.EDSolver(norb, nbath, 1.0, 0.1, np.zeros(ns), np.full((norb, max(nbath, 0)), 0.5), **kw)
+        assert ei.value.code == moed.MOED_EARG, (norb, nbath, kw)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_create_fails_loudly_without_a_device():
+    """with no GPU the library raises MOED_EHIP at moed_create: there is no host fallback"""
+    import numpy as np
+    import moed
+    with pytest.raises(moed.MoedError) as ei:
+        moed.EDSolver(2, 2, 1.0, 0.1, np.zeros(4), np.full((2, 2), 0.5))
+    assert ei.value.code == moed.MOED_EHIP
+
+
 def test_fortran_interface_compiles():
     """the ISO_C_BINDING module compiles with the image's Fortran compiler"""
     import shutil

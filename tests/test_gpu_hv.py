@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
+import moed
 from moed import EDSolver
 from moed.model import load_configs, start_vector
 
@@ -282,14 +283,24 @@ def test_apply_c_matches_reference_build(configs):
 
 
 def test_bad_arguments_raise():
+    """each error class comes back as its MOED_E* code (the reference aborts through die(), sys.F:5-41):
+    a sector out of range, a bad operator, a zero start vector, results asked for before
+    moed_solve, a non-finite bath"""
     rng = np.random.default_rng(0)
     ek, vk, _ = random_model(rng, 2, 2)
-    with EDSolver(2, 2, 1.0, 0.1, ek, vk) as s:
-        with pytest.raises(Exception):
-            s.multiply_H(5, 1, np.zeros(4))
-        with pytest.raises(Exception):
-            s.apply_c(2, 2, np.zeros(36), 3, 1, 1)
-        with pytest.raises(Exception):
-            s.lanczos_iteration(2, 2, np.zeros(36), 10)  # zero start vector
-    with pytest.raises(Exception):
+
+    def code(f, *a):
+        with pytest.raises(moed.MoedError) as ei:
+            f(*a)
+        return ei.value.code
+
+    with EDSolver(2, 2, 1.0, 0.1, ek, vk, sectors=[(2, 2)], nev=1) as s:
+        assert code(s.multiply_H, 5, 1, np.zeros(4)) == moed.MOED_ESECTOR
+        assert code(s.apply_c, 2, 2, np.zeros(36), 3, 1, 1) == moed.MOED_EARG
+        assert code(s.lanczos_iteration, 2, 2, np.zeros(36), 10) == moed.MOED_EZERO
+        assert code(s.ed_calc_green_ftn, moed.omega_grid(8, 10.0)) == moed.MOED_ESTATE
+        assert code(s.sector_eigenvalues, 1) == moed.MOED_ESTATE
+        s.ed_solve()
+        assert code(s.sector_eigenvalues, 2) == moed.MOED_EARG
+    with pytest.raises(moed.MoedError):
         EDSolver(2, 2, 1.0, 0.1, np.array([0, 0, np.nan, 0]), vk)
