@@ -306,8 +306,13 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 for (int c = 0; c < CPL; ++c) acc.v[c] *= inv;
             }
             if (ok) {
-                __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
-                if (CPL == 2 && ok1) __builtin_nontemporal_store(acc.v[CPL - 1], Y + ro + iu + 1);
+                if (CPL == 2 && ok1) {  // one 16-byte store (two 8-byte ones write ~2 % more)
+                    typedef double d2v __attribute__((ext_vector_type(2)));
+                    const d2v v = {acc.v[0], acc.v[CPL - 1]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(Y + ro + iu));
+                } else {
+                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
+                }
                 if (P.dot_part) {
                     dsum = fma(xo.v[0], acc.v[0], dsum);
                     if (CPL == 2 && ok1) dsum = fma(xo.v[CPL - 1], acc.v[CPL - 1], dsum);
