@@ -57,8 +57,8 @@ EXPORTS = [
     "moed_hv", "moed_hv_device", "moed_apply_c", "moed_lanczos", "moed_solve", "moed_sector_eigenvalues",
     "moed_eigvec", "moed_green", "moed_green_nrecords", "moed_green_record", "moed_green_cf",
     "moed_green_cf_adjoint", "moed_calc_dev", "moed_chi2_grad", "moed_minimize", "moed_omega_grid",
-    "moed_fill_start_device", "moed_set_hv_shape", "moed_set_eig_chunk", "moed_stream", "moed_band_tb", "moed_delta_new",
-    "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
+    "moed_fill_start_device", "moed_set_hv_shape", "moed_set_eig_chunk", "moed_stream", "moed_band_tb",
+    "moed_delta_new", "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
     "moed_export_eigvec", "moed_import_eigvec", "moed_export_green", "moed_import_green",
 ]
 
