@@ -135,6 +135,11 @@ module moed_capi
             type(c_ptr), value :: h
             integer(c_int32_t), value :: iterations
         end function
+        integer(c_int) function moed_set_eig_chunk(h, vectors) bind(C, name="moed_set_eig_chunk")
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: vectors
+        end function
         integer(c_int) function moed_fit_nfev(h, nfit, nfev) bind(C, name="moed_fit_nfev")
             import :: c_ptr, c_int, c_int32_t
             type(c_ptr), value :: h
