@@ -1558,8 +1558,15 @@ int moed_import_green(moed_handle* h, const char* path) {
         std::memcpy(&r.E, q, 8), q += 8;
         std::memcpy(&r.p, q, 8), q += 8;
         std::memcpy(&i4, q, 4), q += 4, r.even = i4 != 0;
+        // moed_green_record hands out nstep + 1 coefficients per array: longer records do not fit
+        if (r.iorb < 1 || r.iorb > h->norb || r.nsp > h->nstep || r.nsn > h->nstep) {
+            ok = false;
+            break;
+        }
+        // every array holds nstep + 1 entries, zero past the record's own steps (as green_diag
+        // leaves the entries past a breakdown, ed_lanczos.F90:27-28)
         auto take = [&](std::vector<double>& v, int n) {
-            v.resize(n);
+            v.assign((size_t)h->nstep + 1, 0.0);
             std::memcpy(v.data(), q, 8 * (size_t)n);
             q += 8 * (size_t)n;
         };
@@ -1567,11 +1574,6 @@ int moed_import_green(moed_handle* h, const char* path) {
         take(r.bp, r.nsp + 1);
         take(r.an, r.nsn + 1);
         take(r.bn, r.nsn + 1);
-        // moed_green_record hands out nstep + 1 coefficients per array: longer records do not fit
-        if (r.iorb < 1 || r.iorb > h->norb || r.nsp > h->nstep || r.nsn > h->nstep) {
-            ok = false;
-            break;
-        }
         // the file does not store the spin: per state, Norb spin-up records, then (N_up != N_dn)
         // Norb spin-down records (ed_green.F90:107-131)
         count_in_iev = r.iev == last_iev ? count_in_iev + 1 : 0;

@@ -75,6 +75,8 @@ def test_apbpanbn_import_and_export_match_the_reference_bytes(tmp_path):
             assert r["ispin"] == (1 if even or i in (2, 3) else 2)
             for k, n in (("ap", nsp), ("bp", nsp), ("an", nsn), ("bn", nsn)):
                 assert np.array_equal(r[k][:n + 1], z[f"rec{i}_{k}"])
+                # records shorter than the handle's nstep come back zero-padded to nstep + 1
+                assert n < 10 and np.all(r[k][n + 1:] == 0.0)
         out = tmp_path / "again.dat"
         s.export_green(str(out))
         assert out.read_bytes() == z["ab_bytes"].tobytes()
