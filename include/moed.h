@@ -180,6 +180,18 @@ int moed_chi2_grad(moed_handle* h, int32_t nfit, int32_t nx, const double* x, in
  * workgroup per fit; omega must be positive and omega, Dev, x finite. */
 int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t nw, const double* omega,
                   const double* Dev, double ftol, double* fret, int32_t* iter, int32_t* status);
+/* moed_minimize on device buffers, allocating no device memory.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP stream capture, serving loops, harnesses
+ * that forbid hipMalloc).  x_device[nfit][nx] in/out, omega_device[nw], Dev_device (complex
+ * (Norb, nw)) in; fret_device[nfit], iter_device[nfit], status_device[nfit] out (any may be NULL),
+ * all on the handle's device, ordered on `stream` (NULL = the handle's stream); the call returns
+ * when the results are written.  At most 1024 fits per call (MOED_EARG beyond).  The CG state lives
+ * in one fixed device array per GPU (~1.6 MB, held until the process ends) that every call on that
+ * GPU shares: calls that overlap on one GPU race.  Same results as moed_minimize bit for bit;
+ * moed_fit_nfev is not updated. */
+int moed_minimize_device_noalloc(moed_handle* h, int32_t nfit, int32_t nx, double* x_device, int32_t nw,
+                                 const double* omega_device, const double* Dev_device, double ftol,
+                                 double* fret_device, int32_t* iter_device, int32_t* status_device, void* stream);
 /* CG iterations each fit runs per kernel launch inside moed_minimize (0: default).  It bounds the
  * duration of one launch; results do not depend on it.  No counterpart in the reference. */
 int moed_set_fit_launch(moed_handle* h, int32_t iterations);

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <string>
 
 struct moed_handle;
@@ -25,6 +26,16 @@ int fit_eval(const FitCtx& c, int nfit, int nx, const double* x, int nw, const d
 int fit_delta(const FitCtx& c, int nx, const double* x, int nw, const double* omega, double* Dev);
 int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
                  double ftol, double* fret, int* iter, int* status);
+
+// Allocation-free minimisation: x (in/out), omega, Dev and the outputs are device pointers, the CG
+// state of at most kNoAllocFits fits lives in a file-scope device array whose address on the current
+// device fit_scratch_address returns (looked up once per device).  No device allocation or free.
+constexpr int kNoAllocFits = 1024;
+constexpr int kMaxDevices = 64;
+int fit_scratch_address(int device, void** out);
+int fit_minimize_noalloc(const FitCtx& c, void* scratch_addr, int nfit, int nx, double* x, int nw,
+                         const double* omega, const double* Dev, double ftol, double* fret, int32_t* iter,
+                         int32_t* status);
 
 }  // namespace moed
 

@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "moed_cplx.h"
@@ -841,6 +843,42 @@ int fit_delta(const FitCtx& c, int nx, const double* x, int nw, const double* om
     return MOED_OK;
 }
 
+namespace {
+
+__global__ void fit_zero_i32_kernel(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t zero_i32(int32_t* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fit_zero_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// The CG launches of one minimisation over device buffers: st / ist hold each fit's state between
+// launches (ist zero = not started), act counts the fits a launch leaves running.  Every launch
+// ends each running fit after `budget` more iterations; kItmax bounds the loop.
+int fit_run(const FitCtx& c, int nfit, int nx, double* dx, int nw, const double* dom, const double* ddev,
+            double ftol, double* st, int32_t* ist, int32_t* act) {
+    FCHK(zero_i32(reinterpret_cast<int32_t*>(st), nfit * kStateD * 2, c.stream));
+    FCHK(zero_i32(ist, nfit * kStateI, c.stream));
+    const int budget = c.budget > 0 ? c.budget : (c.tree ? 1024 : 128);
+    FitCgArgs a{make_prob(c, nx, nw, dom, ddev), nfit, budget, ftol, dx, st, ist, act};
+    for (int launch = 0; launch <= kItmax; ++launch) {
+        FCHK(zero_i32(act, 1, c.stream));
+        hipLaunchKernelGGL(fit_cg_kernel, dim3(nfit), dim3(kFitThreads), 0, c.stream, a);
+        FCHK(hipGetLastError());
+        int32_t active = 0;
+        FCHK(hipMemcpyAsync(&active, act, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+        FCHK(hipStreamSynchronize(c.stream));
+        if (active == 0) break;
+    }
+    return MOED_OK;
+}
+
+}  // namespace
+
 int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const double* omega, const double* Dev,
                  double ftol, double* fret, int* iter, int* status) {
     int rc = check_problem(c, nfit, nx, x, nw, omega, Dev);
@@ -855,21 +893,9 @@ int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const dou
     FCHK(hipMemcpyAsync(dx.p, x, (size_t)nfit * nx * sizeof(double), hipMemcpyHostToDevice, c.stream));
     FCHK(hipMemcpyAsync(dom.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
     FCHK(hipMemcpyAsync(ddev.p, Dev, (size_t)2 * c.norb * nw * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    FCHK(hipMemsetAsync(dst.p, 0, (size_t)nfit * kStateD * sizeof(double), c.stream));
-    FCHK(hipMemsetAsync(dist.p, 0, (size_t)nfit * kStateI * sizeof(int32_t), c.stream));
-    const int budget = c.budget > 0 ? c.budget : (c.tree ? 1024 : 128);
-    FitCgArgs a{make_prob(c, nx, nw, dom.as<double>(), ddev.as<double>()), nfit, budget, ftol, dx.as<double>(),
-                dst.as<double>(), dist.as<int32_t>(), dact.as<int32_t>()};
-    // every launch ends each running fit after `budget` more iterations; kItmax bounds the loop
-    for (int launch = 0; launch <= kItmax; ++launch) {
-        FCHK(hipMemsetAsync(dact.p, 0, sizeof(int32_t), c.stream));
-        hipLaunchKernelGGL(fit_cg_kernel, dim3(nfit), dim3(kFitThreads), 0, c.stream, a);
-        FCHK(hipGetLastError());
-        int32_t active = 0;
-        FCHK(hipMemcpyAsync(&active, dact.p, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-        FCHK(hipStreamSynchronize(c.stream));
-        if (active == 0) break;
-    }
+    rc = fit_run(c, nfit, nx, dx.as<double>(), nw, dom.as<double>(), ddev.as<double>(), ftol, dst.as<double>(),
+                 dist.as<int32_t>(), dact.as<int32_t>());
+    if (rc) return rc;
     std::vector<double> stv((size_t)nfit * kStateD);
     std::vector<int32_t> isv((size_t)nfit * kStateI);
     FCHK(hipMemcpyAsync(x, dx.p, (size_t)nfit * nx * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -886,6 +912,105 @@ int fit_minimize(const FitCtx& c, int nfit, int nx, double* x, int nw, const dou
         if (iter) iter[i] = s[1];
         if (status) status[i] = s[2];
         if (c.nfev) c.nfev[i] = s[3];
+    }
+    return MOED_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free minimisation
+//
+// The CG state of up to kNoAllocFits fits lives in one file-scope device array, so that a caller
+// who cannot allocate device memory (code under stream capture, serving loops) can still run the
+// minimiser on its own device buffers.  Every GPU holds its own copy; its address is looked up once
+// per device (fit_scratch_address, called when a handle is created) and only that address is
+// handed to kernels and copies.  The array keeps the last call's contents, so each call clears the
+// parts it reads before use.  Calls that overlap on one GPU share the array and race.
+
+namespace {
+
+struct NoAllocScratch {
+    double st[(size_t)kNoAllocFits * kStateD];
+    int32_t ist[(size_t)kNoAllocFits * kStateI];
+    int32_t flag[64];  // [0] fits a launch leaves running, [1] input check bits, [2] unfinished fits
+};
+
+static __device__ __align__(256) NoAllocScratch scratch;  // ~1.6 MB per GPU
+
+// input checks of check_problem on device data: bit 0 omega not positive or not finite, bit 1 Dev
+// not finite, bit 2 x not finite
+__global__ void fit_check_kernel(const double* omega, int nw, const double* dev, int ndev, const double* x, int nxall,
+                                 int32_t* flag) {
+    const int stride = gridDim.x * blockDim.x;
+    int bad = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride)
+        if (!(omega[i] > 0.0) || !isfinite(omega[i])) bad |= 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndev; i += stride)
+        if (!isfinite(dev[i])) bad |= 2;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nxall; i += stride)
+        if (!isfinite(x[i])) bad |= 4;
+    if (bad) atomicOr(flag, bad);
+}
+
+// per fit: fret, iterations, status; unfinished fits are counted into *unfinished
+__global__ void fit_results_kernel(const double* st, const int32_t* ist, int nfit, double* fret, int32_t* iter,
+                                   int32_t* status, int32_t* unfinished) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfit) return;
+    const int32_t* s = ist + (size_t)i * kStateI;
+    if (s[0] != kPhaseDone) atomicAdd(unfinished, 1);
+    if (fret) fret[i] = st[(size_t)i * kStateD + 3 * kFitNx + 1];
+    if (iter) iter[i] = s[1];
+    if (status) status[i] = s[2];
+}
+
+}  // namespace
+
+int fit_scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return MOED_EARG;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        // the device must be current: every GPU has its own copy of the array
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MOED_EHIP;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return MOED_OK;
+}
+
+int fit_minimize_noalloc(const FitCtx& c, void* scratch_addr, int nfit, int nx, double* x, int nw,
+                         const double* omega, const double* Dev, double ftol, double* fret, int32_t* iter,
+                         int32_t* status) {
+    if (nfit > kNoAllocFits) {
+        if (c.err) *c.err = "minimize_device_noalloc: at most " + std::to_string(kNoAllocFits) + " fits per call";
+        return MOED_EARG;
+    }
+    if (nx > kFitNx || c.nbath > kFitMaxBath || c.norb > kMaxOrb) return argfail(c, "fit: nx or Nbath too large");
+    NoAllocScratch* sc = static_cast<NoAllocScratch*>(scratch_addr);
+    int32_t* flag = sc->flag;
+    FCHK(zero_i32(flag, 64, c.stream));
+    const int ndev = 2 * c.norb * nw, nxall = nfit * nx;
+    const int nb = std::min(1024, (std::max({nw, ndev, nxall}) + 255) / 256);
+    hipLaunchKernelGGL(fit_check_kernel, dim3(nb), dim3(256), 0, c.stream, omega, nw, Dev, ndev, x, nxall, flag + 1);
+    FCHK(hipGetLastError());
+    int32_t bad = 0;
+    FCHK(hipMemcpyAsync(&bad, flag + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipStreamSynchronize(c.stream));
+    if (bad & 1) return argfail(c, "fit: omega must be positive");
+    if (bad & 2) return argfail(c, "fit: omega and Dev must be finite");
+    if (bad & 4) return argfail(c, "fit: x must be finite");
+    int rc = fit_run(c, nfit, nx, x, nw, omega, Dev, ftol, sc->st, sc->ist, flag);
+    if (rc) return rc;
+    hipLaunchKernelGGL(fit_results_kernel, dim3((nfit + 255) / 256), dim3(256), 0, c.stream, sc->st, sc->ist, nfit,
+                       fret, iter, status, flag + 2);
+    FCHK(hipGetLastError());
+    int32_t unfinished = 0;
+    FCHK(hipMemcpyAsync(&unfinished, flag + 2, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    FCHK(hipStreamSynchronize(c.stream));
+    if (unfinished) {
+        if (c.err) *c.err = "fit: a fit did not finish";
+        return MOED_ECONV;
     }
     return MOED_OK;
 }

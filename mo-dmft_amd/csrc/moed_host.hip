@@ -131,6 +131,7 @@ struct moed_handle {
     int hv_threads = 0, hv_strip = 0;
     int fit_budget = 0;            // CG iterations per fit and launch of moed_minimize (0: default)
     std::vector<int32_t> fit_nfev;  // function evaluations of each fit of the last moed_minimize
+    void* fit_scratch = nullptr;    // this device's copy of the allocation-free minimiser's state array
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
@@ -1032,8 +1033,14 @@ int moed_create(const moed_params* p, moed_handle** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e != hipSuccess) {
+        if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
+    }
+    if (fit_scratch_address(h->device, &h->fit_scratch) != MOED_OK) {  // device is current
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return MOED_EHIP;
     }
     int rc = set_model(h, p->ek, p->vk);
     if (rc) {
@@ -1638,6 +1645,20 @@ int moed_minimize(moed_handle* h, int32_t nfit, int32_t nx, double* x, int32_t n
     if (!h || !x || !omega || !Dev || nfit < 1 || nw < 1) return MOED_EARG;
     if (nx != h->nbath * (1 + h->norb) + h->norb) return fail(h, MOED_EARG, "nx != Nbath*(1+Norb)+Norb");
     return fit_minimize_host(h, nfit, nx, x, nw, omega, Dev, ftol, fret, iter, status);
+}
+
+int moed_minimize_device_noalloc(moed_handle* h, int32_t nfit, int32_t nx, double* x_device, int32_t nw,
+                                 const double* omega_device, const double* Dev_device, double ftol,
+                                 double* fret_device, int32_t* iter_device, int32_t* status_device, void* stream) {
+    if (!h) return MOED_EARG;
+    if (!x_device || !omega_device || !Dev_device || nfit < 1 || nw < 1)
+        return fail(h, MOED_EARG, "minimize_device_noalloc: bad arguments");
+    if (nx != h->nbath * (1 + h->norb) + h->norb) return fail(h, MOED_EARG, "nx != Nbath*(1+Norb)+Norb");
+    (void)hipSetDevice(h->device);
+    FitCtx c = fit_ctx(h);
+    if (stream) c.stream = (hipStream_t)stream;
+    return fit_minimize_noalloc(c, h->fit_scratch, nfit, nx, x_device, nw, omega_device, Dev_device, ftol,
+                                fret_device, iter_device, status_device);
 }
 
 int moed_band_tb(int32_t norb, int32_t nq, double* hk) {

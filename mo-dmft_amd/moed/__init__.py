@@ -60,6 +60,7 @@ EXPORTS = [
     "moed_fill_start_device", "moed_set_hv_shape", "moed_set_eig_chunk", "moed_stream", "moed_band_tb",
     "moed_delta_new", "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
     "moed_export_eigvec", "moed_import_eigvec", "moed_export_green", "moed_import_green",
+    "moed_minimize_device_noalloc",
 ]
 
 _lib = None
@@ -97,6 +98,9 @@ def lib():
         L.moed_chi2_grad.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, _dp, ct.c_int32, _dp, _dp, _dp, _dp]
         L.moed_minimize.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, _dp, ct.c_int32, _dp, _dp, ct.c_double,
                                     _dp, _ip, _ip]
+        L.moed_minimize_device_noalloc.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_int32,
+                                                   ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_void_p, ct.c_void_p,
+                                                   ct.c_void_p, ct.c_void_p]
         L.moed_omega_grid.argtypes = [ct.c_int32, ct.c_double, _dp]
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
