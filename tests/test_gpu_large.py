@@ -1,0 +1,73 @@
+"""GPU parity at the three large workloads against fixtures written in the build container by
+tests/golden/make_large.py (the oracle, SciPy's ARPACK and the reference build):
+
+  * 10-site model, all 121 sectors, 8 eigenpairs each, beta = 50, 1024 frequencies: every sector's
+    eigenvalues, the kept states (ed_solver.F90:65-90) and G(i w_n) (ed_green.F90:60-185), sector by
+    sector and with every sector in one multi-problem run;
+  * 14-site sector (7, 7): E0 and G(i w_n) of both orbitals on 2048 frequencies (200 steps);
+  * 15-site three-orbital sector (7, 8) with t_ij (D = 41,409,225): E0, and the particle Green's
+    function of orbital 1, spin up, on 4096 frequencies;
+  * the bath fit at the 15-site shape (nx 51, 4096 frequencies) in the reference's summation order:
+    the reference build's minimization result bit for bit, with the oracle's iteration and
+    FUNC-evaluation counts.
+
+Tolerances: eigenvalues 1e-10 relative (1e-10 absolute below |E| = 1), G 1e-8 relative to max |G|,
+kept-state probabilities 1e-8; the fit is bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+import moed
+from moed import EDSolver
+from moed.model import config
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+BIG = np.finfo(float).max
+
+
+def _gerr(G, Gref):
+    return float(np.max(np.abs(G - Gref)) / np.max(np.abs(Gref)))
+
+
+@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
+def test_ns10_all_sectors_eigenvalues_states_and_green(flags):
+    z = np.load(os.path.join(GOLD, "ns10_batched.npz"))
+    c = config("ns10_batched")
+    om = z["omega"]
+    print(f"ns10_batched: 121 sectors, nev {c.nev}, flags {flags}", flush=True)
+    with EDSolver.from_config(c, flags=flags) as s:
+        kept = s.ed_solve()
+        ev = np.array([s.sector_eigenvalues(i + 1) for i in range(len(c.sectors))])
+        G = s.ed_calc_green_ftn(om)
+    ref = z["ev"]
+    fin = ref < BIG
+    assert np.array_equal(ev >= BIG, ~fin)  # the same levels exist (nev > D leaves DBL_MAX)
+    err = np.abs(ev[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))
+    print(f"  eigenvalues: max err {err.max():.2e} over {fin.sum()} levels", flush=True)
+    assert err.max() <= 1e-10
+    # kept states: the spin-partner sectors are exactly degenerate, so their order in the sort is
+    # set by rounding; compare as sets keyed by (sector, level)
+    kr = {(int(r[2]), int(r[3])): (r[0], r[1]) for r in z["kept"]}
+    kg = {(int(k[2]), int(k[3])): (k[0], k[1]) for k in kept}
+    assert kg.keys() == kr.keys()
+    for key, (E, p) in kr.items():
+        assert abs(kg[key][0] - E) <= 1e-10 * max(1.0, abs(E))
+        assert abs(kg[key][1] - p) <= 1e-8
+    e = _gerr(G, z["G"])
+    print(f"  G: rel err {e:.2e}", flush=True)
+    assert e <= 1e-8
+
+
+def test_ns15_shape_fit_reference_order_bit_exact():
+    z = np.load(os.path.join(GOLD, "ns15_fit.npz"))
+    norb, nbath = int(z["norb"]), int(z["nbath"])
+    print(f"ns15 fit: nx {len(z['x0'])}, {len(z['omega'])} frequencies, reference order", flush=True)
+    with EDSolver(norb, nbath, 0.0, 0.0, np.zeros(norb + nbath), np.zeros((norb, nbath))) as s:
+        x, fret, it, st = s.minimization(z["x0"], z["omega"], z["Dev"])
+        nfev = s.fit_nfev(1)[0]
+    print(f"  fret {fret[0]:.6e} iter {it[0]} nfev {nfev}", flush=True)
+    assert st[0] == 0
+    assert np.array_equal(x[0], z["xmin"]) and fret[0] == float(z["fret"])
+    assert it[0] == int(z["iter"]) and nfev == int(z["nfev"])
