@@ -338,6 +338,138 @@ __global__ void __launch_bounds__(kEigThreads) eig_project_kernel(EigArgs e, con
     if (threadIdx.x == 0) e.part[item] = ss;
 }
 
+// ------------------------------------------------------------------ FP64 MFMA contractions
+//
+// The whole-basis passes of full reorthogonalisation and the Ritz product on the matrix cores,
+// v_mfma_f64_16x16x4_f64: D(16x16) += A(16x4) B(4x16) with A[m][k] in lane m + 16k, B[k][n] in
+// lane n + 16k and D[i / 16 + 4j][i % 16] in element j of lane i's accumulator.  Against one w the
+// Gram-Schmidt contractions are matrix-vector products, so one of the 16 columns carries data; the
+// passes stay bound by HBM (8 bytes of basis per multiply-add) with the matrix cores well below
+// their rate.  The Ritz product uses nev of the 16 columns.
+typedef double mf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf4 mfma16(double a, double b, mf4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// part[i * ni + item] = <V_i, w> over the item for the 16 basis vectors g0 <= i < g0 + 16 (i < k1)
+// of group blockIdx.y.  Lane (m, q) = (lane % 16, lane / 16) holds elements 4q..4q+3 of each
+// 16-element slice of vector g0 + m (two 16-byte loads); the lanes of column 0 hold the same
+// elements of w.  Element j of the four goes to accumulator j: four independent MFMA chains.
+__global__ void __launch_bounds__(kEigThreads) eig_multidot_mfma_kernel(EigArgs e, const double* const* V,
+                                                                        const double* w, int i0, int k1, int pass) {
+    __shared__ double red[kEigThreads / 64][16];
+    const int item = blockIdx.x;
+    const int p = e.item_prob[item];
+    if (!eig_active(e, p, pass)) return;
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int g0 = i0 + blockIdx.y * 16;
+    const bool have = g0 + m < k1;
+    const double* __restrict__ u = have ? V[g0 + m] + base : w + base;  // (a valid address either way)
+    const double* __restrict__ wb = w + base;
+    mf4 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = mf4{0.0, 0.0, 0.0, 0.0};
+    constexpr int kPerWave = kEigChunk / (kEigThreads / 64);  // 256 elements per wave
+    const int r0 = wid * kPerWave + 4 * q;
+    // every item starts on an even element, so 16-byte loads of a whole item are aligned
+    if (len == kEigChunk && ((reinterpret_cast<uintptr_t>(u) | reinterpret_cast<uintptr_t>(wb)) & 15) == 0) {
+#pragma unroll 2
+        for (int r = r0; r < r0 + kPerWave; r += 16) {
+            const double2 a0 = *reinterpret_cast<const double2*>(u + r);
+            const double2 a1 = *reinterpret_cast<const double2*>(u + r + 2);
+            const double2 b0 = *reinterpret_cast<const double2*>(wb + r);
+            const double2 b1 = *reinterpret_cast<const double2*>(wb + r + 2);
+            const double z = 0.0;
+            acc[0] = mfma16(have ? a0.x : z, m == 0 ? b0.x : z, acc[0]);
+            acc[1] = mfma16(have ? a0.y : z, m == 0 ? b0.y : z, acc[1]);
+            acc[2] = mfma16(have ? a1.x : z, m == 0 ? b1.x : z, acc[2]);
+            acc[3] = mfma16(have ? a1.y : z, m == 0 ? b1.y : z, acc[3]);
+        }
+    } else {
+        for (int r = r0; r < r0 + kPerWave; r += 16) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool in = r + j < len;
+                acc[j] = mfma16((have && in) ? u[r + j] : 0.0, (m == 0 && in) ? wb[r + j] : 0.0, acc[j]);
+            }
+        }
+    }
+    // column 0: lane 16 q holds D[q + 4 j][0] = <V_{g0 + q + 4 j}, w> over this wave's elements
+    if (m == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[wid][q + 4 * j] = (acc[0][j] + acc[1][j]) + (acc[2][j] + acc[3][j]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int i = g0 + threadIdx.x;
+        if (i < k1) {
+            double sum = 0.0;
+            for (int v = 0; v < kEigThreads / 64; ++v) sum += red[v][threadIdx.x];
+            e.part[(size_t)i * e.ni + item] = sum;
+        }
+    }
+}
+
+// dst = src - sum_{i0 <= i < k1} c_i V_i over the item; part[item] = |dst|^2 over the item.
+// Per 16-element slice: B[k][n] = V_{i + k}[slice + n] (lanes n + 16 k, 128 contiguous bytes per
+// vector), A[0][k] = c_{i + k} (rows 1..15 zero), so lanes 0..15 end with D[0][n] in element 0.
+// A wave works on four slices at once (four independent MFMA chains); the basis addresses of the
+// pass sit in LDS.
+__global__ void __launch_bounds__(kEigThreads) eig_project_mfma_kernel(EigArgs e, const double* const* V,
+                                                                       const double* src, double* dst, int i0,
+                                                                       int k1, int pass) {
+    extern __shared__ double sc[];  // c_i, then the addresses of V_i (i < k1)
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x;
+    const int p = e.item_prob[item];
+    if (!eig_active(e, p, pass)) return;
+    const double** vp = reinterpret_cast<const double**>(sc + k1);
+    for (int i = threadIdx.x; i < k1; i += kEigThreads) {
+        sc[i] = e.c[(size_t)p * e.ldh + i];
+        vp[i] = V[i];
+    }
+    __syncthreads();
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int n = lane & 15, k = lane >> 4;
+    constexpr int kPerWave = kEigChunk / (kEigThreads / 64);
+    double ss = 0.0;
+    for (int s0 = wid * kPerWave; s0 < (wid + 1) * kPerWave; s0 += 64) {
+        mf4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = mf4{0.0, 0.0, 0.0, 0.0};
+        for (int i = i0; i < k1; i += 4) {
+            const int ii = i + k;
+            const bool have = ii < k1;
+            const double* __restrict__ v = (have ? vp[ii] : src) + base + s0 + n;
+            double b[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) b[t] = (have && s0 + 16 * t + n < len) ? v[16 * t] : 0.0;
+            const double a = (n == 0 && have) ? sc[ii] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma16(a, b[t], acc[t]);
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = s0 + 16 * t + n;
+                if (r < len) {
+                    const double v = src[base + r] - acc[t][0];
+                    dst[base + r] = v;
+                    ss = fma(v, v, ss);
+                }
+            }
+        }
+    }
+    ss = block_sum<kEigThreads>(ss, red);
+    if (threadIdx.x == 0) e.part[item] = ss;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -411,10 +543,10 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
         wn[k] = bk > 0.0 ? rt / bk : 0.0;
         wn[k + 1] = 1.0;
         aux[1] = wp2;
-        const bool need = bk > 0.0 && (mx > sqrt(eps) || e.force[p]);
+        const bool need = bk > 0.0 && (e.full_mode || mx > sqrt(eps) || e.force[p]);
         if (need) {
             e.full[p] = 1;
-            e.force[p] = e.force[p] ? 0 : 1;
+            e.force[p] = (e.force[p] || e.full_mode) ? 0 : 1;
         } else {
             e.full[p] = 0;
             eig_close_step(e, p, k, wp2, tiny);
@@ -423,7 +555,8 @@ __global__ void __launch_bounds__(64) eig_finish_kernel(EigArgs e, int k, int pa
     }
     if (lane != 0) return;
     aux[2] = wp2;
-    if (pass == 1 && wp2 < eta2 * aux[1]) {  // second pass: w' lost more than half its norm
+    // second pass: w' lost more than half its norm (DGKS), or always with full reorthogonalisation
+    if (pass == 1 && (e.full_mode || wp2 < eta2 * aux[1])) {
         e.pass2[p] = 1;
         aux[1] = wp2;
         return;
@@ -445,11 +578,108 @@ hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1
     double* dst = vk1;
     const int nv = k1 - i0;
     const int ng = (nv + kEigVG - 1) / kEigVG;
-    hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, i0, k1, pass);
+    if (e.full_mode && pass > 0) {  // CGS2 against the whole basis on the matrix cores
+        hipLaunchKernelGGL(eig_multidot_mfma_kernel, dim3(e.ni, (nv + 15) / 16), dim3(kEigThreads), 0, s, e, V, src,
+                           i0, k1, pass);
+    } else {
+        hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, V, src, i0, k1, pass);
+    }
     hipLaunchKernelGGL(eig_reduce_h_kernel, dim3(e.P, (nv + 3) / 4), dim3(kEigThreads), 0, s, e, i0, k1, pass);
-    hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * sizeof(double), s, e, V, src,
-                       dst, i0, k1, pass);
+    if (e.full_mode && pass > 0) {
+        hipLaunchKernelGGL(eig_project_mfma_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * 2 * sizeof(double), s,
+                           e, V, src, dst, i0, k1, pass);
+    } else {
+        hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)k1 * sizeof(double), s, e, V,
+                           src, dst, i0, k1, pass);
+    }
     hipLaunchKernelGGL(eig_finish_kernel, dim3(e.P), dim3(64), 0, s, e, k, pass, eta2, tiny);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ many reference recurrences
+//
+// lanczos_iteration (ed_lanczos.F90:37-68) for several chains at once, packed like the eigensolver's
+// problems: per chain p the scalars A[p][0..], B[p][0..] (B = squared norms of the unnormalised
+// vectors u_j), a stop flag and the step count at breakdown.  Per step: u_j from w = H u_{j-1} and
+// the two previous vectors (one launch for every chain), B[p][j] = |u_j|^2, w = H u_j, A[p][j] =
+// <u_j, w> / B[p][j], breakdown when B[p][j] < thr for j >= 2 (a chain that stops is skipped).
+
+// out = (w - A[j-1] u1) / sqrt(B[j-1]) + c3 u0 over the item, c3 = -sqrt(B[j-1] / B[j-2]) (j >= 2);
+// part[item] = |out|^2 over the item.  The same operations per element as lz_update_kernel.
+__global__ void __launch_bounds__(kEigThreads) lzb_update_kernel(EigArgs e, int j, const double* w, const double* u1,
+                                                                 const double* u0, double* out, const double* A,
+                                                                 const double* B, int ldh, const int32_t* stop) {
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x;
+    const int p = e.item_prob[item];
+    if (stop[p]) return;
+    const double* Bp = B + (size_t)p * ldh;
+    const double sb = sqrt(Bp[j - 1]);
+    const double c1 = 1.0 / sb, a = A[(size_t)p * ldh + j - 1];
+    const double c3 = (j >= 2) ? -(sb / sqrt(Bp[j - 2])) : 0.0;
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    double s = 0.0;
+    for (int r = threadIdx.x; r < len; r += kEigThreads) {
+        const int64_t i = base + r;
+        double v = (w[i] - a * u1[i]) * c1;
+        if (j >= 2) v += c3 * u0[i];
+        out[i] = v;
+        s += v * v;
+    }
+    s = block_sum<kEigThreads>(s, red);
+    if (threadIdx.x == 0) e.part[item] = s;
+}
+
+// one wave per chain: which = 0: B[p][j] = sum of the item partials; which = 1: A[p][j] = sum / B[p][j]
+// and the breakdown test (j >= 2 and B[p][j] < thr: stop, nsc = j)
+__global__ void __launch_bounds__(64) lzb_finish_kernel(EigArgs e, int j, int which, double* A, double* B, int ldh,
+                                                        int32_t* stop, int32_t* nsc, double thr) {
+    const int p = blockIdx.x;
+    if (stop[p]) return;
+    const double sum = wave_item_sum(e.part, e.pitem0[p], e.pnitem[p]);
+    if (threadIdx.x != 0) return;
+    double* Bp = B + (size_t)p * ldh;
+    if (which == 0) {
+        Bp[j] = sum;
+        return;
+    }
+    A[(size_t)p * ldh + j] = sum / Bp[j];
+    if (j >= 2 && Bp[j] < thr) {
+        stop[p] = 1;
+        nsc[p] = j;
+    }
+}
+
+hipError_t launch_lzb_update(const EigArgs& e, int j, const double* w, const double* u1, const double* u0,
+                             double* out, const double* A, const double* B, int ldh, const int32_t* stop,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(lzb_update_kernel, dim3(e.ni), dim3(kEigThreads), 0, s, e, j, w, u1, u0, out, A, B, ldh, stop);
+    return hipGetLastError();
+}
+
+hipError_t launch_lzb_finish(const EigArgs& e, int j, int which, double* A, double* B, int ldh, int32_t* stop,
+                             int32_t* nsc, double thr, hipStream_t s) {
+    hipLaunchKernelGGL(lzb_finish_kernel, dim3(e.P), dim3(64), 0, s, e, j, which, A, B, ldh, stop, nsc, thr);
+    return hipGetLastError();
+}
+
+// part[item] = <a, b> over the item, for chains that have not stopped
+__global__ void __launch_bounds__(kEigThreads) lzb_dot_kernel(EigArgs e, const double* a, const double* b,
+                                                              const int32_t* stop) {
+    __shared__ double red[kEigThreads / 64];
+    const int item = blockIdx.x;
+    if (stop[e.item_prob[item]]) return;
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    double s = 0.0;
+    for (int r = threadIdx.x; r < len; r += kEigThreads) s = fma(a[base + r], b[base + r], s);
+    s = block_sum<kEigThreads>(s, red);
+    if (threadIdx.x == 0) e.part[item] = s;
+}
+
+hipError_t launch_lzb_dot(const EigArgs& e, const double* a, const double* b, const int32_t* stop, hipStream_t s) {
+    hipLaunchKernelGGL(lzb_dot_kernel, dim3(e.ni), dim3(kEigThreads), 0, s, e, a, b, stop);
     return hipGetLastError();
 }
 
@@ -497,6 +727,72 @@ __global__ void __launch_bounds__(kEigThreads) eig_ritz_kernel(EigArgs e, const 
     if (threadIdx.x == 0) e.part[(size_t)j * e.ni + item] = s2;
 }
 
+// Ritz vectors on the matrix cores: out[p] + j dim_p = sum_{i < m_p} S_{j,i} V_i over the item for
+// the 16 Ritz vectors j = 16 blockIdx.y + n (n < 16, j < nf[p]); part[j * ni + item] = |y_j|^2.
+// Per 16-element slice and 4 basis vectors: A[m][k] = V_{i+k}[slice + m], B[k][n] = S_{j,i+k}.
+__global__ void __launch_bounds__(kEigThreads) eig_ritz_mfma_kernel(EigArgs e, const double* const* V,
+                                                                    const double* S, int lds, int nev,
+                                                                    const int32_t* nf, double* const* out) {
+    extern __shared__ double ss[];  // [16][m]
+    __shared__ double red[kEigThreads / 64][16];
+    const int item = blockIdx.x, j0 = blockIdx.y * 16;
+    const int p = e.item_prob[item];
+    const int nj = min(16, nf[p] - j0);
+    if (nj <= 0) return;
+    const int m = e.mfinal[p];
+    for (int t = threadIdx.x; t < 16 * m; t += kEigThreads) {
+        const int n = t / m, i = t % m;
+        ss[t] = n < nj ? S[((size_t)p * nev + j0 + n) * lds + i] : 0.0;
+    }
+    __syncthreads();
+    const int64_t base = e.item_start[item];
+    const int len = e.item_len[item];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lm = lane & 15, lk = lane >> 4;  // A row / B column, and the k index
+    constexpr int kPerWave = kEigChunk / (kEigThreads / 64);
+    double s2 = 0.0;  // |y_n|^2 over this lane's elements, n = lane % 16
+    const int64_t ybase = base - e.poff[p];
+    double* y = lm < nj ? out[p] + (size_t)(j0 + lm) * e.pdim[p] + ybase : nullptr;
+    for (int s0 = wid * kPerWave; s0 < (wid + 1) * kPerWave; s0 += 64) {  // four slices, four MFMA chains
+        mf4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = mf4{0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < m; i += 4) {
+            const int ii = i + lk;
+            const bool have = ii < m;
+            const double* __restrict__ v = V[have ? ii : 0] + base + s0 + lm;
+            double a[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = (have && s0 + 16 * t + lm < len) ? v[16 * t] : 0.0;
+            const double b = have ? ss[lm * m + ii] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma16(a[t], b, acc[t]);
+        }
+        if (y) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rr = s0 + 16 * t + lk + 4 * j;
+                    if (rr < len) {
+                        y[rr] = acc[t][j];
+                        s2 = fma(acc[t][j], acc[t][j], s2);
+                    }
+                }
+        }
+    }
+    // lanes n, n + 16, n + 32, n + 48 hold parts of |y_n|^2
+    s2 += __shfl_xor(s2, 16, 64);
+    s2 += __shfl_xor(s2, 32, 64);
+    if (lane < 16) red[wid][lane] = s2;
+    __syncthreads();
+    if (threadIdx.x < nj) {
+        double t = 0.0;
+        for (int v = 0; v < kEigThreads / 64; ++v) t += red[v][threadIdx.x];
+        e.part[(size_t)(j0 + threadIdx.x) * e.ni + item] = t;
+    }
+}
+
 __global__ void __launch_bounds__(64) eig_ritz_norm_kernel(EigArgs e, int nev, const int32_t* nf, double* n2) {
     const int p = blockIdx.x, j = blockIdx.y;
     if (j >= nf[p]) return;
@@ -518,8 +814,19 @@ __global__ void __launch_bounds__(kEigThreads) eig_ritz_scale_kernel(EigArgs e, 
 
 hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s) {
-    hipLaunchKernelGGL(eig_ritz_kernel, dim3(e.ni, nev), dim3(kEigThreads), (size_t)lds * sizeof(double), s, e, V, S,
-                       lds, nev, nf, out);
+    if (e.full_mode) {
+        const size_t shm = (size_t)16 * lds * sizeof(double);
+        if (shm > 64 * 1024) {
+            hipError_t err = hipFuncSetAttribute((const void*)eig_ritz_mfma_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (err != hipSuccess) return err;
+        }
+        hipLaunchKernelGGL(eig_ritz_mfma_kernel, dim3(e.ni, (nev + 15) / 16), dim3(kEigThreads), shm, s, e, V, S, lds,
+                           nev, nf, out);
+    } else {
+        hipLaunchKernelGGL(eig_ritz_kernel, dim3(e.ni, nev), dim3(kEigThreads), (size_t)lds * sizeof(double), s, e, V,
+                           S, lds, nev, nf, out);
+    }
     hipLaunchKernelGGL(eig_ritz_norm_kernel, dim3(e.P, nev), dim3(64), 0, s, e, nev, nf, n2);
     hipLaunchKernelGGL(eig_ritz_scale_kernel, dim3(e.ni, nev), dim3(kEigThreads), 0, s, e, nev, nf, out, n2);
     return hipGetLastError();

@@ -140,6 +140,8 @@ struct EigArgs {
     double* om;                // [P][3][ldh] estimated overlaps omega_{k-1}, omega_k, omega_{k+1} (rotating)
     int32_t* full;             // [P] 1: the current step projects against the whole basis
     int32_t* force;            // [P] 1: the next step does too (the step after a triggered one)
+    int32_t full_mode;         // 1: every step projects twice against the whole basis (CGS2) on the
+                               // matrix cores, and the Ritz product runs there too
 };
 // part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
@@ -156,6 +158,16 @@ hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1
 // Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
 hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s);
+
+// many reference recurrences at once (the Green's-function chains), packed like the eigensolver's
+// problems; A, B, stop, nsc per chain (A, B with row length ldh)
+hipError_t launch_lzb_update(const EigArgs& e, int j, const double* w, const double* u1, const double* u0,
+                             double* out, const double* A, const double* B, int ldh, const int32_t* stop,
+                             hipStream_t s);
+// which = 0: B[p][j] = sum of part over p's items; 1: A[p][j] = sum / B[p][j] and the breakdown test
+hipError_t launch_lzb_finish(const EigArgs& e, int j, int which, double* A, double* B, int ldh, int32_t* stop,
+                             int32_t* nsc, double thr, hipStream_t s);
+hipError_t launch_lzb_dot(const EigArgs& e, const double* a, const double* b, const int32_t* stop, hipStream_t s);
 
 // apply_c: per target string, the source rank and sign of the elementary operator
 struct ApplyCArgs {
@@ -212,6 +224,26 @@ struct CfArgs {
     int32_t stride;
 };
 hipError_t launch_cf(const CfArgs& a, hipStream_t s);
+// the continued fractions of many chains: G(o, w_i) += weight_c g_c(w_i) over the chains c of
+// orbital o in chain order (the order of the sequential accumulation).  Chain c: coefficients
+// A[c][0..nsc[c]], B[c][1..nsc[c]] (row length ldh) with b0[c] in place of B[c][0], sign[c] (+1
+// particle, -1 hole), E[c], weight[c]; chains with use[c] == 0 contribute nothing.
+struct CfBatchArgs {
+    int32_t nchain, ldh, norb;
+    const double* A;
+    const double* B;
+    const double* b0;
+    const int32_t* nsc;
+    const int32_t* sign;
+    const int32_t* iorb;  // 0-based
+    const int32_t* use;
+    const double* E;
+    const double* weight;
+    const double* omega;
+    int32_t nw, compat;
+    double* G;  // interleaved complex (norb, nw) column-major
+};
+hipError_t launch_cf_batch(const CfBatchArgs& a, hipStream_t s);
 
 // real-axis spectral function of one record (dos_diag, dos.f90:65-106):
 // out[i * stride] = -Im(G+ + G-)(w_i) / pi, or out += weight * that when accumulate
