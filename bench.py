@@ -18,7 +18,7 @@ the wall time of each stage of each iteration:
 With --solve CONFIG it times one impurity solve (ed_solve, then ed_calc_green_ftn) of a named
 workload, all of its sectors in one multi-problem run with --batched:
 
-    python bench.py --solve ns14_roofline | ns10_batched [--batched]
+    python bench.py --solve ns14_roofline | ns10_batched [--batched] [--full-reorth]
 """
 import argparse
 import json
@@ -188,7 +188,7 @@ def solve_bench(args):
     from moed.model import config, omega_grid
 
     cfg = config(args.solve)
-    flags = moed.MOED_F_BATCHED if args.batched else 0
+    flags = (moed.MOED_F_BATCHED if args.batched else 0) | (moed.MOED_F_FULL_REORTH if args.full_reorth else 0)
     om = omega_grid(cfg.nw, cfg.beta)
     with moed.EDSolver.from_config(cfg, flags=flags) as s:
         t0 = time.perf_counter()
@@ -199,7 +199,8 @@ def solve_bench(args):
     out = {"metric": "impurity solve wall s (ed_solve + ed_calc_green_ftn)", "value": round(t2 - t0, 4), "unit": "s",
            "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{cfg.name}: {cfg.nsite} sites, {len(cfg.sectors)} sector(s), nev {cfg.nev}, "
-                                  f"nstep {cfg.nstep}, {cfg.nw} frequencies", "batched": bool(args.batched)},
+                                  f"nstep {cfg.nstep}, {cfg.nw} frequencies", "batched": bool(args.batched),
+                      "reorthogonalisation": "full (CGS2, FP64 MFMA)" if args.full_reorth else "partial"},
            "t_solve_s": round(t1 - t0, 4), "t_green_s": round(t2 - t1, 4), "E0": kept[0][0], "states": len(kept)}
     print(json.dumps(out))
 
@@ -223,6 +224,8 @@ def main():
     ap.add_argument("--fit-order", choices=("tree", "reference"), default="tree")
     ap.add_argument("--solve", default=None, help="time one impurity solve of this workload instead")
     ap.add_argument("--batched", action="store_true", help="with --solve: all sectors in one multi-problem run")
+    ap.add_argument("--full-reorth", action="store_true",
+                    help="with --solve: full reorthogonalisation (CGS2 on the FP64 matrix cores) in the eigensolver")
     args = ap.parse_args()
     if args.dmft:
         return dmft_bench(args)
@@ -254,39 +257,52 @@ def main():
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
+    t2 = torch.cuda.Event(enable_timing=True)
+    t3 = torch.cuda.Event(enable_timing=True)
+
+    def hv_step():
+        solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
+
     for _ in range(args.warmup):
-        solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
-    # eager: one library call per step, queued right behind the warm-up steps
-    t0.record(ts)
-    for _ in range(args.steps):
-        solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
-    t1.record(ts)
-    torch.cuda.synchronize()
-    ms_eager = t0.elapsed_time(t1) / args.steps
-    # the same K steps captured once in a HIP graph (no host launch gaps between the kernels)
-    ms_graph = None
+        hv_step()
+    # the K steps captured once in a HIP graph (no host launch gaps between the kernels)
+    g = None
     try:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=ts):
             for _ in range(args.steps):
-                solver.multiply_H_device(nu, nd, x.data_ptr(), y.data_ptr(), stream)
-        torch.cuda.synchronize()
-        # The timed replay is queued right behind an untimed warm replay of the same graph: after a
-        # host synchronisation leaves the GPU idle, the next ~3 ms of kernels measured 10-15 % slower
-        # in kernel traces (every kernel of the replay, not launch gaps), so the K timed steps start
-        # on a busy GPU.  The events bracket exactly the K timed steps.
-        wall0 = time.perf_counter()
-        with torch.cuda.stream(ts):
-            g.replay()  # warm replay, untimed
-            t0.record(ts)
-            g.replay()
-            t1.record(ts)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - wall0  # both replays (2 K steps)
-        ms_graph = t0.elapsed_time(t1) / args.steps
+                hv_step()
     except Exception as e:  # capture unsupported: report the eager figure
         print(f"bench.py: graph capture failed ({e}); timing the eager loop", file=sys.stderr)
-        wall = ms_eager * args.steps / 1e3
+        g = None
+    torch.cuda.synchronize()
+    # Clock ramp: kernel traces of consecutive K-step loops (eager and graph alike) run 125-135 us
+    # per H v in the first few milliseconds after the GPU was idle and settle near 110 us after
+    # ~40 ms of sustained load.  So untimed H v steps keep the GPU loaded for at least WARM_MS
+    # before the timed steps, which are queued directly behind them with no host synchronisation
+    # in between; events bracket exactly the K timed steps of each form.
+    WARM_MS = 150.0
+    n_warm = max(1, math.ceil(WARM_MS / (0.12 * args.steps)))
+    wall0 = time.perf_counter()
+    with torch.cuda.stream(ts):
+        for _ in range(n_warm):
+            if g is not None:
+                g.replay()
+            else:
+                for _ in range(args.steps):
+                    hv_step()
+        t0.record(ts)
+        for _ in range(args.steps):  # eager: one library call per step
+            hv_step()
+        t1.record(ts)
+        if g is not None:
+            t2.record(ts)
+            g.replay()
+            t3.record(ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0  # warm-up load plus both timed forms
+    ms_eager = t0.elapsed_time(t1) / args.steps
+    ms_graph = t2.elapsed_time(t3) / args.steps if g is not None else None
     ms = ms_graph if ms_graph is not None else ms_eager
     with torch.cuda.stream(ts):
         for _ in range(args.copy_check):
@@ -312,7 +328,8 @@ def main():
                      "kernel": "one H v = hv_rows_kernel + hv_strip_kernel on one stream"},
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
-        "wall_s_warm_and_timed_replays": round(wall, 4),
+        "wall_s_warm_and_timed": round(wall, 4),
+        "untimed_load_steps": n_warm * args.steps,
         "ms_per_step_eager": round(ms_eager, 6),
         "timing": "HIP graph of the K steps" if ms_graph is not None else "eager loop",
     }

@@ -135,6 +135,7 @@ struct moed_handle {
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
+    DevBuf gbuf[5];  // batched Green's-function chains (green_batch)
     std::vector<std::unique_ptr<DevBuf>> basis;  // its Krylov basis, in chunks of whole vectors
 };
 
@@ -681,6 +682,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     e.om = d_om;
     e.full = d_full;
     e.force = d_force;
+    e.full_mode = (h->flags & MOED_F_FULL_REORTH) ? 1 : 0;
 
     // basis vectors: those the handle's chunks already hold, then new chunks as steps need them
     const size_t vbytes = (size_t)dtot * sizeof(double);
@@ -965,6 +967,273 @@ int green_diag(moed_handle* h, const Sector& s, const double* psi, double E, dou
             rec.an = a;
             rec.bn = b;
         }
+    }
+    return MOED_OK;
+}
+
+// apply_c's map for one chain (as apply_c_dev builds it): per target string of the changed spin,
+// source rank | parity << 30, or -1
+void apply_c_map(int pm, int iorb, int ispin, const Sector& in, const Sector& out, int32_t* map) {
+    const int o = iorb - 1;
+    const StrTable* tin = ispin == 1 ? in.up : in.dn;
+    const StrTable* tout = ispin == 1 ? out.up : out.dn;
+    for (int r = 0; r < tout->count; ++r) {
+        map[r] = -1;
+        const uint32_t s = tout->str[r];
+        const bool occ = (s >> o) & 1u;
+        if ((pm == 1) != occ) continue;
+        const uint32_t src = s ^ (1u << o);
+        int par = __builtin_popcount(s & ((1u << o) - 1u)) & 1;
+        if (ispin == 2) par ^= (in.nu & 1);
+        map[r] = tin->rank[src] | (par << 30);
+    }
+}
+
+// ed_calc_green_ftn (ed_green.F90:60-134) with every chain -- kept state x spin x orbital x (c+, c)
+// -- in one multi-problem run of the reference recurrence: the start vectors c(+)|psi> of all
+// chains packed end to end, one launch per operation for every chain, no host round trip per step
+// (one to read every chain's |c+ psi|^2 before the recurrence, one for the coefficients after it).
+// The continued fractions are then summed into G per orbital in the sequential path's order.
+int green_batch(moed_handle* h, int nw, const double* d_omega, double* d_G) {
+    hipStream_t st = h->stream;
+    const int nstep = h->nstep, norb = h->norb;
+    const int compat = (h->flags & MOED_F_EXACT_CMPLX) ? 0 : 1;
+    struct Chain {
+        int iev, iorb, ispin, pm;
+        double E, weight, p;
+        bool even, valid;
+        Sector src, dst;
+        const double* psi;
+    };
+    std::vector<Chain> ch;
+    for (size_t iev = 0; iev < h->kept_val.size(); ++iev) {
+        const int is = h->kept_sector[iev] - 1, il = h->kept_level[iev] - 1;
+        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+        const bool even = nd == nu;
+        Sector s;
+        if (int rc = get_sector(h, nu, nd, &s)) return rc;
+        const SectorEig& E = *h->eig[is];
+        if (il >= E.nfound) return fail(h, MOED_ESTATE, "kept state has no eigenvector");
+        for (int ispin = 1; ispin <= (even ? 1 : 2); ++ispin)
+            for (int iorb = 1; iorb <= norb; ++iorb)
+                for (int pm = 1; pm <= 2; ++pm) {
+                    Chain c{};
+                    c.iev = (int)iev;
+                    c.iorb = iorb;
+                    c.ispin = ispin;
+                    c.pm = pm;
+                    c.E = h->kept_val[iev];
+                    c.p = h->kept_prob[iev];
+                    c.weight = h->kept_prob[iev] * (even ? 1.0 : 0.5);
+                    c.even = even;
+                    c.src = s;
+                    c.psi = E.vec.as<double>() + (size_t)il * E.D;
+                    int nuo, ndo;
+                    target_sector(pm, ispin, nu, nd, &nuo, &ndo);
+                    c.valid = nuo >= 0 && ndo >= 0 && nuo <= h->nsite && ndo <= h->nsite;
+                    if (c.valid)
+                        if (int rc = get_sector(h, nuo, ndo, &c.dst)) return rc;
+                    ch.push_back(c);
+                }
+    }
+    const int P = (int)ch.size();
+    if (P == 0) return MOED_OK;
+    // packed layout (every chain starts on an even element), items, H v row items, apply_c maps
+    constexpr int64_t kSmall = (int64_t)1 << 20;
+    std::vector<int64_t> off(P, 0), dim(P, 0), item_start;
+    std::vector<int32_t> item_prob, item_len, pitem0(P), pnitem(P), row_prob, row_id, map_off(P, 0);
+    std::vector<BatchProbDev> desc(P);
+    int64_t dtot = 0, nmap = 0;
+    int max_nU = 1;
+    for (int p = 0; p < P; ++p) {
+        Chain& c = ch[p];
+        pitem0[p] = (int32_t)item_prob.size();
+        if (c.valid) {
+            dim[p] = c.dst.D;
+            off[p] = dtot;
+            dtot += dim[p] + (dim[p] & 1);
+            for (int64_t s0 = 0; s0 < dim[p]; s0 += kEigChunk) {
+                item_prob.push_back(p);
+                item_start.push_back(off[p] + s0);
+                item_len.push_back((int32_t)std::min<int64_t>(kEigChunk, dim[p] - s0));
+            }
+            desc[p] = {c.dst.up->dev(), c.dst.dn->dev(), off[p]};
+            if (dim[p] <= kSmall) {
+                for (int id = 0; id < c.dst.dn->count; ++id) {
+                    row_prob.push_back(p);
+                    row_id.push_back(id);
+                }
+                max_nU = std::max(max_nU, c.dst.up->count);
+            }
+            map_off[p] = (int32_t)nmap;
+            nmap += c.ispin == 1 ? c.dst.up->count : c.dst.dn->count;
+        } else {
+            desc[p] = BatchProbDev{};
+        }
+        pnitem[p] = (int32_t)item_prob.size() - pitem0[p];
+    }
+    const int ni = (int)item_prob.size(), nr = (int)row_prob.size();
+    std::vector<int32_t> maps((size_t)std::max<int64_t>(nmap, 1));
+    for (int p = 0; p < P; ++p)
+        if (ch[p].valid) apply_c_map(ch[p].pm, ch[p].iorb, ch[p].ispin, ch[p].src, ch[p].dst, maps.data() + map_off[p]);
+    const int ldh = nstep + 1;
+    // device buffers (handle-owned, grown as needed)
+    DevBuf* g = h->gbuf;
+    HIPCHK(g[0].ensure((size_t)std::max<int64_t>(4 * dtot, 1) * sizeof(double)));  // u0, u1, u2, w
+    const size_t n32 = (size_t)2 * ni + 2 * (size_t)P + 2 * (size_t)nr + 8 * (size_t)P + maps.size();
+    HIPCHK(g[1].ensure(n32 * sizeof(int32_t)));
+    HIPCHK(g[2].ensure(((size_t)ni + 2 * P + 1) * sizeof(int64_t)));
+    const size_t ndbl = 2 * (size_t)P * ldh + (size_t)std::max(ni, 1) + 4 * (size_t)P;
+    HIPCHK(g[3].ensure(ndbl * sizeof(double)));
+    HIPCHK(g[4].ensure((size_t)P * sizeof(BatchProbDev)));
+    double* U[3] = {g[0].as<double>(), g[0].as<double>() + dtot, g[0].as<double>() + 2 * dtot};
+    double* W = g[0].as<double>() + 3 * dtot;
+    int32_t* i32 = g[1].as<int32_t>();
+    int32_t *d_item_prob = i32, *d_item_len = d_item_prob + ni, *d_pitem0 = d_item_len + ni, *d_pnitem = d_pitem0 + P,
+            *d_row_prob = d_pnitem + P, *d_row_id = d_row_prob + nr, *d_stop = d_row_id + nr, *d_nsc = d_stop + P,
+            *d_sign = d_nsc + P, *d_iorb = d_sign + P, *d_use = d_iorb + P, *d_maps = d_use + P;
+    int64_t *d_item_start = g[2].as<int64_t>();
+    double* dbl = g[3].as<double>();
+    double *d_A = dbl, *d_B = d_A + (size_t)P * ldh, *d_part = d_B + (size_t)P * ldh, *d_b0 = d_part + std::max(ni, 1),
+           *d_E = d_b0 + P, *d_w = d_E + P, *d_n2 = d_w + P;
+    if (ni > 0) {
+        HIPCHK(hipMemcpyAsync(d_item_prob, item_prob.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_item_len, item_len.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_item_start, item_start.data(), ni * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d_pitem0, pitem0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_pnitem, pnitem.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (nr > 0) {
+        HIPCHK(hipMemcpyAsync(d_row_prob, row_prob.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_row_id, row_id.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d_maps, maps.data(), maps.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(g[4].p, desc.data(), P * sizeof(BatchProbDev), hipMemcpyHostToDevice, st));
+    HIPCHK(launch_fill(0.0, (int64_t)2 * P * ldh, d_A, st));
+    EigArgs e{};
+    e.P = P;
+    e.ni = ni;
+    e.item_prob = d_item_prob;
+    e.item_start = d_item_start;
+    e.item_len = d_item_len;
+    e.pitem0 = d_pitem0;
+    e.pnitem = d_pnitem;
+    e.part = d_part;
+    // start vectors c(+)|psi> and their squared norms (one host round trip)
+    for (int p = 0; p < P; ++p) {
+        const Chain& c = ch[p];
+        if (!c.valid) continue;
+        ApplyCArgs a{c.psi, U[0] + off[p], d_maps + map_off[p], c.dst.up->count, c.dst.dn->count, c.src.up->count,
+                     c.ispin == 1 ? 0 : 1};
+        HIPCHK(launch_apply_c(a, st));
+    }
+    std::vector<int32_t> zero(P, 0);
+    HIPCHK(hipMemcpyAsync(d_stop, zero.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (ni > 0) HIPCHK(launch_eig_segdot(e, U[0], U[0], d_n2, 1, 0, st));
+    std::vector<double> n2(P, 0.0);
+    HIPCHK(hipMemcpyAsync(n2.data(), d_n2, P * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int32_t> stop(P), nsc(P), sign(P), iorb(P), use(P);
+    std::vector<double> b0(P), Ev(P), wv(P), normsq(P);
+    for (int p = 0; p < P; ++p) {
+        const Chain& c = ch[p];
+        const double v = c.valid ? n2[p] : 0.0;
+        const bool have = c.valid && v > 0.0;  // c(+)|psi> = 0: no contribution (green_diag)
+        // b(0) of green_diag: |c+ psi|^2 for the particle chain, 1 - |c+ psi|^2 for the hole chain
+        // (ed_green.F90:158, :174); the hole chain follows its particle chain
+        normsq[p] = c.pm == 1 ? v : normsq[p - 1];
+        b0[p] = c.pm == 1 ? normsq[p] : 1.0 - normsq[p];
+        stop[p] = have ? 0 : 1;
+        nsc[p] = have ? nstep : 0;
+        use[p] = have;
+        sign[p] = c.pm == 1 ? 1 : -1;
+        iorb[p] = c.iorb - 1;
+        Ev[p] = c.E;
+        wv[p] = c.weight;
+    }
+    HIPCHK(hipMemcpyAsync(d_stop, stop.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_nsc, nsc.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_sign, sign.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_iorb, iorb.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_use, use.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_b0, b0.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_E, Ev.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_w, wv.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+    // H v of every chain: one launch for the sectors up to 2^20 elements, the two-pass product per
+    // larger sector (each skipped once its chain has stopped)
+    HvBatchArgs hb{};
+    hb.prob = g[4].as<BatchProbDev>();
+    hb.row_prob = d_row_prob;
+    hb.row_id = d_row_id;
+    hb.ek = h->d_ek.as<double>();
+    hb.slotc = h->d_slotc.as<double>();
+    hb.U = h->U;
+    hb.UmJ3 = h->UmJ3;
+    hb.Um2J = h->Um2J;
+    hb.J = h->J;
+    hb.skip = d_stop;
+    auto hv_all = [&](const double* x) -> int {
+        hb.x = x;
+        hb.y = W;
+        HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
+        for (int p = 0; p < P; ++p)
+            if (use[p] && dim[p] > kSmall)
+                if (int rc = hv_dev(h, ch[p].dst, x + off[p], W + off[p], nullptr, nullptr, d_stop + p, st)) return rc;
+        return MOED_OK;
+    };
+    if (ni > 0) {
+        // j = 0: B[0] = |u_0|^2, w = H u_0, A[0] = <u_0, w> / B[0]
+        HIPCHK(launch_lzb_dot(e, U[0], U[0], d_stop, st));
+        HIPCHK(launch_lzb_finish(e, 0, 0, d_A, d_B, ldh, d_stop, d_nsc, kBreakdown, st));
+        if (int rc = hv_all(U[0])) return rc;
+        HIPCHK(launch_lzb_dot(e, U[0], W, d_stop, st));
+        HIPCHK(launch_lzb_finish(e, 0, 1, d_A, d_B, ldh, d_stop, d_nsc, kBreakdown, st));
+        for (int j = 1; j <= nstep; ++j) {
+            double* uj = U[j % 3];
+            HIPCHK(launch_lzb_update(e, j, W, U[(j - 1) % 3], U[(j + 1) % 3], uj, d_A, d_B, ldh, d_stop, st));
+            HIPCHK(launch_lzb_finish(e, j, 0, d_A, d_B, ldh, d_stop, d_nsc, kBreakdown, st));
+            if (int rc = hv_all(uj)) return rc;
+            HIPCHK(launch_lzb_dot(e, uj, W, d_stop, st));
+            HIPCHK(launch_lzb_finish(e, j, 1, d_A, d_B, ldh, d_stop, d_nsc, kBreakdown, st));
+        }
+    }
+    CfBatchArgs cf{P, ldh, norb, d_A, d_B, d_b0, d_nsc, d_sign, d_iorb, d_use, d_E, d_w, d_omega, nw, compat, d_G};
+    HIPCHK(launch_cf_batch(cf, st));
+    // the records (apbpanbn.dat order: per state, spin, orbital; particle then hole)
+    std::vector<double> Ah((size_t)P * ldh), Bh((size_t)P * ldh);
+    HIPCHK(hipMemcpyAsync(Ah.data(), d_A, Ah.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(Bh.data(), d_B, Bh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nsc.data(), d_nsc, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    h->records.clear();
+    for (int p = 0; p < P; p += 2) {
+        GreenRecord rec{};
+        rec.iev = ch[p].iev + 1;
+        rec.iorb = ch[p].iorb;
+        rec.ispin = ch[p].ispin;
+        rec.E = ch[p].E;
+        rec.p = ch[p].p;
+        rec.even = ch[p].even;
+        for (int q = p; q < p + 2; ++q) {
+            const int n = use[q] ? nsc[q] : 0;
+            std::vector<double> a(nstep + 1, 0.0), b(nstep + 1, 0.0);
+            if (use[q])
+                for (int j = 0; j <= n; ++j) {
+                    a[j] = Ah[(size_t)q * ldh + j];
+                    b[j] = Bh[(size_t)q * ldh + j];
+                }
+            b[0] = b0[q];
+            if (ch[q].pm == 1) {
+                rec.nsp = n;
+                rec.ap = std::move(a);
+                rec.bp = std::move(b);
+            } else {
+                rec.nsn = n;
+                rec.an = std::move(a);
+                rec.bn = std::move(b);
+            }
+        }
+        h->records.push_back(std::move(rec));
     }
     return MOED_OK;
 }
@@ -1274,6 +1543,26 @@ int moed_green(moed_handle* h, int32_t nw, const double* omega, double* Gr) {
     HIPCHK(hipMemcpyAsync(om.p, omega, nw * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(launch_fill(0.0, (int64_t)2 * norb * nw, G.as<double>(), st));
     h->records.clear();
+    // every chain in one multi-problem run where all target sectors take the one-launch H v (up to
+    // 2^20 elements); larger chains are bandwidth-bound on their own and run one at a time, on
+    // rows padded to an even stride where that lets the H v read 16-byte pairs
+    bool small = true;
+    for (size_t iev = 0; iev < h->kept_val.size() && small; ++iev) {
+        const int is = h->kept_sector[iev] - 1;
+        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+        for (int d = -1; d <= 1; d += 2) {
+            int64_t D1 = 0, D2 = 0;
+            (void)moed_sector_dim(h, std::clamp(nu + d, 0, h->nsite), nd, &D1);
+            (void)moed_sector_dim(h, nu, std::clamp(nd + d, 0, h->nsite), &D2);
+            if (std::max(D1, D2) > ((int64_t)1 << 20)) small = false;
+        }
+    }
+    if ((h->flags & MOED_F_BATCHED) && small) {
+        if (int rc = green_batch(h, nw, om.as<double>(), G.as<double>())) return rc;
+        HIPCHK(hipMemcpyAsync(Gr, G.p, (size_t)2 * norb * nw * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return MOED_OK;
+    }
     for (size_t iev = 0; iev < h->kept_val.size(); ++iev) {
         const int is = h->kept_sector[iev] - 1, il = h->kept_level[iev] - 1;
         const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];

@@ -199,6 +199,22 @@ hipError_t launch_lz_finish_a(int j, const double* part, int npart, LzScal sc, d
 // green_diag's continued fraction (ed_green.F90:160-168, 176-184) at one frequency per thread, with
 // z = cmplx(+-E, w) rounded as the reference's default-kind cmplx() when compat is set, and
 // G += weight * g in the reference's complex arithmetic.
+// g(z) of green_diag's recurrence over A[0..n], B[0..n] at frequency w (sign +1: z - a_j, -1: z + a_j)
+__device__ __forceinline__ cxd cf_eval(const double* A, const double* B, int n, int sign, double E, double w,
+                                       int compat) {
+    const double Es = sign > 0 ? E : -E;
+    const cxd z = {compat ? round_f32(Es) : Es, compat ? round_f32(w) : w};
+    cxd g;
+    if (sign > 0) {
+        g = cx_rdiv(B[n], {z.re - A[n], z.im});
+        for (int j = n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re - A[j]) - g.re, z.im - g.im});
+    } else {
+        g = cx_rdiv(B[n], {z.re + A[n], z.im});
+        for (int j = n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re + A[j]) - g.re, z.im - g.im});
+    }
+    return g;
+}
+
 __global__ void cf_kernel(CfArgs a) {
     extern __shared__ double sab[];  // a[0..n], b[0..n]
     for (int k = threadIdx.x; k <= a.n; k += blockDim.x) {
@@ -208,22 +224,49 @@ __global__ void cf_kernel(CfArgs a) {
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nw) return;
-    const double* A = sab;
-    const double* B = sab + a.n + 1;
-    const double E = a.sign > 0 ? a.E : -a.E;
-    const cxd z = {a.compat ? round_f32(E) : E, a.compat ? round_f32(a.omega[i]) : a.omega[i]};
-    cxd g;
-    if (a.sign > 0) {
-        g = cx_rdiv(B[a.n], {z.re - A[a.n], z.im});
-        for (int j = a.n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re - A[j]) - g.re, z.im - g.im});
-    } else {
-        g = cx_rdiv(B[a.n], {z.re + A[a.n], z.im});
-        for (int j = a.n - 1; j >= 0; --j) g = cx_rdiv(B[j], {(z.re + A[j]) - g.re, z.im - g.im});
-    }
+    const cxd g = cf_eval(sab, sab + a.n + 1, a.n, a.sign, a.E, a.omega[i], a.compat);
     double* G = a.G + (size_t)2 * i * a.stride;
     const cxd wg = cx_mul({a.weight, 0.0}, g);
     G[0] = G[0] + wg.re;
     G[1] = G[1] + wg.im;
+}
+
+// one thread per (frequency, orbital); the chains of the orbital in chain order, each chain's
+// coefficients staged in LDS by the whole block in turn
+__global__ void __launch_bounds__(256) cf_batch_kernel(CfBatchArgs a) {
+    extern __shared__ double sab[];  // A[0..ldh), B[0..ldh) of the current chain
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int o = blockIdx.y;
+    double* G = a.G + (size_t)2 * ((size_t)o + (size_t)a.norb * (i < a.nw ? i : 0));
+    double gre = i < a.nw ? G[0] : 0.0, gim = i < a.nw ? G[1] : 0.0;
+    for (int c = 0; c < a.nchain; ++c) {
+        if (a.iorb[c] != o || !a.use[c]) continue;  // uniform over the block
+        const int n = a.nsc[c];
+        __syncthreads();
+        for (int k = threadIdx.x; k <= n; k += blockDim.x) {
+            sab[k] = a.A[(size_t)c * a.ldh + k];
+            sab[a.ldh + k] = k == 0 ? a.b0[c] : a.B[(size_t)c * a.ldh + k];
+        }
+        __syncthreads();
+        if (i < a.nw) {
+            const cxd g = cf_eval(sab, sab + a.ldh, n, a.sign[c], a.E[c], a.omega[i], a.compat);
+            const cxd wg = cx_mul({a.weight[c], 0.0}, g);
+            gre = gre + wg.re;
+            gim = gim + wg.im;
+        }
+    }
+    if (i < a.nw) {
+        G[0] = gre;
+        G[1] = gim;
+    }
+}
+
+hipError_t launch_cf_batch(const CfBatchArgs& a, hipStream_t s) {
+    if (a.nchain <= 0) return hipSuccess;
+    const size_t lds = (size_t)2 * a.ldh * sizeof(double);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cf_batch_kernel, dim3((a.nw + 255) / 256, a.norb), dim3(256), lds, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_cf(const CfArgs& a, hipStream_t s) {

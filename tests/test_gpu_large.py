@@ -31,7 +31,7 @@ def _gerr(G, Gref):
     return float(np.max(np.abs(G - Gref)) / np.max(np.abs(Gref)))
 
 
-@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
+@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED, moed.MOED_F_BATCHED | moed.MOED_F_FULL_REORTH])
 def test_ns10_all_sectors_eigenvalues_states_and_green(flags):
     z = np.load(os.path.join(GOLD, "ns10_batched.npz"))
     c = config("ns10_batched")
