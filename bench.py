@@ -191,6 +191,8 @@ def solve_bench(args):
     flags = (moed.MOED_F_BATCHED if args.batched else 0) | (moed.MOED_F_FULL_REORTH if args.full_reorth else 0)
     om = omega_grid(cfg.nw, cfg.beta)
     with moed.EDSolver.from_config(cfg, flags=flags) as s:
+        if args.persistent is not None:
+            s.set_persistent(args.persistent)
         t0 = time.perf_counter()
         kept = s.ed_solve()
         t1 = time.perf_counter()
@@ -224,6 +226,8 @@ def main():
     ap.add_argument("--fit-order", choices=("tree", "reference"), default="tree")
     ap.add_argument("--solve", default=None, help="time one impurity solve of this workload instead")
     ap.add_argument("--batched", action="store_true", help="with --solve: all sectors in one multi-problem run")
+    ap.add_argument("--persistent", type=int, default=None,
+                    help="with --solve --batched: largest sector of the persistent launch (0: none)")
     ap.add_argument("--full-reorth", action="store_true",
                     help="with --solve: full reorthogonalisation (CGS2 on the FP64 matrix cores) in the eigensolver")
     args = ap.parse_args()

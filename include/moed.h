@@ -224,6 +224,11 @@ int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x_d
  * (0 = default 1024, else 256, 512 or 1024) and strip pass width in columns (0 = default 128,
  * 64 or 128; 64 where the up-string count is odd).  Results are bit-identical for every shape. */
 int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip);
+/* sectors of at most max_dim elements that moed_solve with MOED_F_BATCHED runs in its persistent
+ * work-queue launch (no counterpart in the reference): the whole Lanczos iteration of each such
+ * sector inside one workgroup, the sectors pulled from a queue, largest first; larger sectors run
+ * step by step, concurrently.  -1 = default, 0 = none, at most 2^20. */
+int moed_set_persistent(moed_handle* h, int64_t max_dim);
 /* vectors per allocation of the eigensolver's Krylov basis (no counterpart in the reference):
  * 0 = default (what fits in 4 GiB).  The basis is allocated as the iteration reaches it, up to
  * gs_maxvec + 1 vectors, and the handle keeps it for later calls; results do not depend on the

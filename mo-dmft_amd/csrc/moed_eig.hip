@@ -18,7 +18,9 @@
 
 #include "moed_device.h"
 #include "moed_hvops.h"
+#include "moed_hvrow.h"
 #include "moed_internal.h"
+#include "moed_ritzconv.h"
 
 namespace moed {
 
@@ -26,8 +28,6 @@ namespace moed {
 
 template <int NORB, int NW>
 __global__ void __launch_bounds__(kEigThreads) hv_batch_kernel(HvBatchArgs A) {
-    constexpr int KMAX = 2 * NW - 5;
-    constexpr int NP = NORB * NORB;
     extern __shared__ double xs[];
     __shared__ double ekd[kMaxSite];
 
@@ -44,72 +44,13 @@ __global__ void __launch_bounds__(kEigThreads) hv_batch_kernel(HvBatchArgs A) {
         for (int q = 0; q < (int)(sizeof(BatchProbDev) / 8); ++q) wds[q] = src[q];
         __builtin_memcpy(&d, wds, sizeof(d));
     }
-    const int nU = d.up.count, nD = d.dn.count, neg = nU + 1;
-    const int tid = threadIdx.x;
     const double* __restrict__ X = A.x + d.off;
     double* __restrict__ Y = A.y + d.off;
-    const double* __restrict__ xr = X + (size_t)id * nU;
-
-    for (int k = tid; k < nU; k += kEigThreads) {
-        const double v = xr[k];
-        xs[k] = v;
-        xs[neg + k] = -v;
-    }
-    const uint32_t sd = scalar_u16(d.dn.str, id);
-    const int nd = __builtin_popcount(sd);
-    if (tid == 0) {
-        xs[nU] = 0.0;
-        xs[2 * nU + 1] = 0.0;
-        const cptr<double> ekc = as_const(A.ek);
-        uint32_t m = sd;
-        for (int q = 0; m; m &= m - 1, ++q) ekd[q] = ekc[__builtin_ctz(m)];
-    }
+    hv_row_stage(d, id, X, xs, ekd, A.ek, threadIdx.x, kEigThreads);
     __syncthreads();
-
-    const double U = A.U, UmJ3 = A.UmJ3, Um2J = A.Um2J, J = A.J;
-    const cptr<double> slotc = as_const(A.slotc);
-    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(d.up.rec);
-    const int rstride = 4 * d.up.nq;
-    const cptr<uint32_t> mvp = as_const(d.dn.mv);
-    const cptr<double> mvcp = as_const(d.dn.mvc);
-    const int nh = as_const(d.dn.nhop)[id];
-    const size_t m0 = (size_t)id * d.dn.mh;
-    int32_t msf[NP > 1 ? NP : 1], mpe[NP > 1 ? NP : 1];
-    if constexpr (NORB > 1) {
-        const cptr<int32_t> dimpc = as_const(d.dn.imp);
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            const int i = q / NORB, j = q % NORB;
-            msf[q] = i != j ? dimpc[(size_t)(j * NORB + i) * nD + id] : -1;
-            mpe[q] = i != j ? dimpc[(size_t)(i * NORB + j) * nD + id] : -1;
-        }
-    }
     const double inv = A.scale ? 1.0 / A.scale[(size_t)p * A.scale_stride] : 1.0;
-
-    for (int iu = tid; iu < nU; iu += kEigThreads) {
-        uint32_t w[NW];
-#pragma unroll
-        for (int q = 0; q < NW; ++q) w[q] = recw[(size_t)iu * rstride + q];
-        const double Sup = __hiloint2double((int)w[1], (int)w[0]);
-        double acc = __dmul_rn(diag_element<NORB>(Sup, w[2] & 0xffffu, sd, ekd, nd, U, UmJ3, Um2J), xs[iu]);
-#pragma unroll
-        for (int q = 0; q < KMAX; ++q) acc = fma(slotc[q], xs[rec_u16(w, 5 + q)], acc);
-        for (int k = 0; k < nh; ++k) acc = fma(mvcp[m0 + k], X[(size_t)mvp[m0 + k] * nU + iu], acc);
-        if constexpr (NORB > 1) {
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                if (q / NORB == q % NORB) continue;
-                const int32_t mu = d.up.imp[(size_t)q * nU + iu];
-                if (mu >= 0) {
-                    const int uu = mu & 0xffff;
-                    const double cu = (mu >> 16) ? -J : J;
-                    if (msf[q] >= 0) acc = fma((msf[q] >> 16) ? -cu : cu, X[(size_t)(msf[q] & 0xffff) * nU + uu], acc);
-                    if (mpe[q] >= 0) acc = fma((mpe[q] >> 16) ? -cu : cu, X[(size_t)(mpe[q] & 0xffff) * nU + uu], acc);
-                }
-            }
-        }
-        Y[(size_t)id * nU + iu] = A.scale ? acc * inv : acc;
-    }
+    hv_row_compute<NORB, NW>(d, id, X, Y, xs, ekd, A.slotc, A.U, A.UmJ3, A.Um2J, A.J, A.scale != nullptr, inv,
+                             threadIdx.x, kEigThreads);
 }
 
 template <int NORB, int NW>
@@ -377,7 +318,6 @@ __global__ void __launch_bounds__(kEigThreads) eig_multidot_mfma_kernel(EigArgs 
     const int r0 = wid * kPerWave + 4 * q;
     // every item starts on an even element, so 16-byte loads of a whole item are aligned
     if (len == kEigChunk && ((reinterpret_cast<uintptr_t>(u) | reinterpret_cast<uintptr_t>(wb)) & 15) == 0) {
-#pragma unroll 2
         for (int r = r0; r < r0 + kPerWave; r += 16) {
             const double2 a0 = *reinterpret_cast<const double2*>(u + r);
             const double2 a1 = *reinterpret_cast<const double2*>(u + r + 2);
@@ -680,6 +620,29 @@ __global__ void __launch_bounds__(kEigThreads) lzb_dot_kernel(EigArgs e, const d
 
 hipError_t launch_lzb_dot(const EigArgs& e, const double* a, const double* b, const int32_t* stop, hipStream_t s) {
     hipLaunchKernelGGL(lzb_dot_kernel, dim3(e.ni), dim3(kEigThreads), 0, s, e, a, b, stop);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ convergence on the device
+
+// After step m - 1 (basis size m) of the problems still running: the lowest nev Ritz pairs of T_m
+// pass |beta_m z_{m,i}| <= tol |theta_i| (ritz_converged) -> done, mfinal = m.  One wave per problem.
+__global__ void __launch_bounds__(64) eig_conv_kernel(EigArgs e, int m, int nev, double tol, double* zs) {
+    extern __shared__ double t3[];  // 3 ldh: T_m staged
+    const int p = blockIdx.x;
+    if (e.done[p] || m < nev) return;
+    const int ldh = e.ldh;
+    const bool conv = ritz_converged(e.alpha + (size_t)p * ldh, e.beta + (size_t)p * ldh, m, nev, tol, t3, t3 + ldh,
+                                     t3 + 2 * ldh, zs + (size_t)p * nev * 3 * ldh, ldh);
+    if (threadIdx.x == 0 && conv) {
+        e.done[p] = 1;
+        e.mfinal[p] = m;
+    }
+}
+
+hipError_t launch_eig_conv(const EigArgs& e, int m, int nev, double tol, double* zs, hipStream_t s) {
+    const size_t lds = (size_t)3 * e.ldh * sizeof(double);
+    hipLaunchKernelGGL(eig_conv_kernel, dim3(e.P), dim3(64), lds, s, e, m, nev, tol, zs);
     return hipGetLastError();
 }
 

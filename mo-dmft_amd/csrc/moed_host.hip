@@ -132,6 +132,11 @@ struct moed_handle {
     int fit_budget = 0;            // CG iterations per fit and launch of moed_minimize (0: default)
     std::vector<int32_t> fit_nfev;  // function evaluations of each fit of the last moed_minimize
     void* fit_scratch = nullptr;    // this device's copy of the allocation-free minimiser's state array
+    int32_t* pinned_i32 = nullptr;  // pinned host buffer for flags read back by the eigensolver
+    hipStream_t stream2 = nullptr;  // second stream (default priority): the persistent eigensolver
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int64_t persist_max = -1;       // largest sector dimension for the persistent launch (-1: default)
+    size_t pinned_n = 0;
     int ncu = 0;  // compute units of the device
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
@@ -568,7 +573,11 @@ enum { EG_VT = 0, EG_W, EG_I32, EG_I64, EG_DBL, EG_DESC, EG_OUT, EG_N };
 // of 0.3 s.  Kernels address the vectors through a device table of their addresses.
 constexpr size_t kBasisBytes = (size_t)4 << 30;
 
-int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out) {
+// persistent: problems small enough for one workgroup (D <= kPersistMaxDim, at most kPersistMaxU up
+// strings) run their whole iteration in one persistent work-queue launch (moed_persist.hip) instead
+// of step by step; the rest go step by step as before.
+int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out,
+              bool persistent = false) {
     hipStream_t st = h->stream;
     const int P = (int)secs.size();
     if (P == 0) return MOED_OK;
@@ -595,6 +604,21 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     std::vector<int64_t> item_start;
     std::vector<BatchProbDev> desc(P);
     int max_nU = 1;
+    // problems for the persistent work-queue launch, largest first
+    std::vector<char> pers(P, 0);
+    std::vector<int32_t> pqueue;
+    int pers_nU = 1, pers_M = 1;
+    const int64_t pmax = std::min<int64_t>(h->persist_max >= 0 ? h->persist_max : kPersistDefaultDim, kPersistMaxDim);
+    if (persistent && M + 2 <= kPersistMaxBasis) {
+        for (int p = 0; p < P; ++p)
+            if (dim[p] <= pmax && S[p].up->count <= kPersistMaxU) {
+                pers[p] = 1;
+                pqueue.push_back(p);
+                pers_nU = std::max(pers_nU, S[p].up->count);
+                pers_M = std::max(pers_M, (int)mmax[p]);
+            }
+        std::stable_sort(pqueue.begin(), pqueue.end(), [&](int a, int b) { return dim[a] > dim[b]; });
+    }
     for (int p = 0; p < P; ++p) {
         pitem0[p] = (int32_t)item_prob.size();
         for (int64_t s0 = 0; s0 < dim[p]; s0 += kEigChunk) {
@@ -604,6 +628,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         }
         pnitem[p] = (int32_t)item_prob.size() - pitem0[p];
         desc[p] = {S[p].up->dev(), S[p].dn->dev(), off[p]};
+        if (pers[p]) continue;  // its rows are the persistent kernel's
         if (dim[p] <= kSmall) {
             for (int id = 0; id < S[p].dn->count; ++id) {
                 row_prob.push_back(p);
@@ -618,10 +643,11 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     const size_t npart = std::max<size_t>((size_t)ldh * ni, (size_t)nev * ni);
     HIPCHK(g[EG_VT].ensure((size_t)(M + 1) * sizeof(double*)));
     HIPCHK(g[EG_W].ensure((size_t)dtot * sizeof(double)));
-    const size_t n32 = 2 * (size_t)ni + 9 * (size_t)P + 2 * (size_t)nr;
+    const size_t n32 = 2 * (size_t)ni + 9 * (size_t)P + 2 * (size_t)nr + pqueue.size() + 2;
     HIPCHK(g[EG_I32].ensure(n32 * sizeof(int32_t)));
-    HIPCHK(g[EG_I64].ensure(((size_t)ni + 2 * P) * sizeof(int64_t)));
-    const size_t ndbl = 8 * (size_t)P * ldh + 4 * (size_t)P + npart + (size_t)P * nev * ldh + (size_t)P * nev;
+    HIPCHK(g[EG_I64].ensure(((size_t)ni + 3 * P) * sizeof(int64_t)));
+    const size_t nzs = (size_t)P * nev * 3 * ldh;  // convergence test workspace
+    const size_t ndbl = 8 * (size_t)P * ldh + 4 * (size_t)P + npart + (size_t)P * nev * ldh + (size_t)P * nev + nzs;
     HIPCHK(g[EG_DBL].ensure(ndbl * sizeof(double)));
     HIPCHK(g[EG_DESC].ensure((size_t)P * sizeof(BatchProbDev)));
     HIPCHK(g[EG_OUT].ensure((size_t)P * sizeof(double*)));
@@ -629,20 +655,26 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     int32_t *d_item_prob = i32, *d_item_len = d_item_prob + ni, *d_pitem0 = d_item_len + ni, *d_pnitem = d_pitem0 + P,
             *d_mmax = d_pnitem + P, *d_done = d_mmax + P, *d_pass2 = d_done + P, *d_mfinal = d_pass2 + P,
             *d_nf = d_mfinal + P, *d_row_prob = d_nf + P, *d_row_id = d_row_prob + nr, *d_full = d_row_id + nr,
-            *d_force = d_full + P;
-    int64_t *d_item_start = g[EG_I64].as<int64_t>(), *d_poff = d_item_start + ni, *d_pdim = d_poff + P;
+            *d_force = d_full + P, *d_pqueue = d_force + P, *d_head = d_pqueue + pqueue.size();
+    int64_t *d_item_start = g[EG_I64].as<int64_t>(), *d_poff = d_item_start + ni, *d_pdim = d_poff + P,
+            *d_seed = d_pdim + P;
     double* dbl = g[EG_DBL].as<double>();
     double *d_alpha = dbl, *d_beta = d_alpha + (size_t)P * ldh, *d_nrm = d_beta + (size_t)P * ldh,
            *d_h = d_nrm + (size_t)P * ldh, *d_c = d_h + (size_t)P * ldh, *d_om = d_c + (size_t)P * ldh,
            *d_aux = d_om + 3 * (size_t)P * ldh,
-           *d_part = d_aux + 4 * (size_t)P, *d_S = d_part + npart, *d_n2 = d_S + (size_t)P * nev * ldh;
+           *d_part = d_aux + 4 * (size_t)P, *d_S = d_part + npart, *d_n2 = d_S + (size_t)P * nev * ldh,
+           *d_zs = d_n2 + (size_t)P * nev;
     std::vector<int32_t> zeroP(P, 0);
     HIPCHK(hipMemcpyAsync(d_item_prob, item_prob.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_item_len, item_len.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pitem0, pitem0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pnitem, pnitem.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_mmax, mmax.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_done, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    // the problems of the persistent launch count as done for the step-by-step kernels from the
+    // start (the persistent kernel keeps its own flags and sets done and mfinal when it finishes)
+    std::vector<int32_t> done0(P);
+    for (int p = 0; p < P; ++p) done0[p] = pers[p];
+    HIPCHK(hipMemcpyAsync(d_done, done0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pass2, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_mfinal, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_full, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -715,8 +747,9 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     if (int rc = reserve(2)) return rc;
     double* W = g[EG_W].as<double>();
     for (int p = 0; p < P; ++p)
-        HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p], vp[0] + off[p],
-                                 st));
+        if (!pers[p])
+            HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p],
+                                     vp[0] + off[p], st));
     HIPCHK(launch_eig_segdot(e, vp[0], vp[0], d_nrm, ldh, 1, st));  // nrm[p][0] = |u_0|
 
     HvBatchArgs hb{};
@@ -735,7 +768,51 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
 
     std::vector<int32_t> done_h(P, 0), mfin_h(P, 0);
     std::vector<double> al_h((size_t)P * ldh), be_h((size_t)P * ldh);
-    int active = P;
+    if (h->pinned_n < (size_t)P) {
+        if (h->pinned_i32) (void)hipHostFree(h->pinned_i32);
+        h->pinned_i32 = nullptr;
+        h->pinned_n = 0;
+        HIPCHK(hipHostMalloc((void**)&h->pinned_i32, (size_t)P * sizeof(int32_t), hipHostMallocDefault));
+        h->pinned_n = (size_t)P;
+    }
+    for (int p = 0; p < P; ++p) h->pinned_i32[p] = pers[p];
+    int active = P - (int)pqueue.size();
+    if (!pqueue.empty()) {
+        // the whole iteration of every queued problem in one launch; its basis is allocated up front
+        if (int rc = reserve(pers_M + 1)) return rc;
+        std::vector<uint64_t> seeds(P);
+        for (int p = 0; p < P; ++p) seeds[p] = 0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second;
+        const int32_t zero2[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(d_pqueue, pqueue.data(), pqueue.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_head, zero2, sizeof(zero2), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_seed, seeds.data(), P * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        PersistArgs pa{};
+        pa.e = e;
+        pa.prob = g[EG_DESC].as<BatchProbDev>();
+        pa.V = vt;
+        pa.W = W;
+        pa.ek = h->d_ek.as<double>();
+        pa.slotc = h->d_slotc.as<double>();
+        pa.U = h->U;
+        pa.UmJ3 = h->UmJ3;
+        pa.Um2J = h->Um2J;
+        pa.J = h->J;
+        pa.seed = reinterpret_cast<const uint64_t*>(d_seed);
+        pa.order = d_pqueue;
+        pa.nq = (int32_t)pqueue.size();
+        pa.head = d_head;
+        pa.nev = nev;
+        pa.max_nU = pers_nU;
+        pa.tol = h->gs_tol;
+        pa.zs = d_zs;
+        const int grid = std::max(1, std::min((int)pqueue.size(), h->ncu));
+        // on the second stream, concurrent with the step-by-step iteration of the larger problems
+        // (the two touch disjoint rows of every table and disjoint ranges of every vector)
+        HIPCHK(hipEventRecord(h->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        HIPCHK(launch_eig_persistent(pa, grid, h->norb, (int)h->slots.size(), h->stream2));
+        HIPCHK(hipEventRecord(h->ev_join, h->stream2));
+    }
     for (int k = 0; k < M && active > 0; ++k) {
         if (int rc = reserve(k + 2)) return rc;  // step k writes vector k + 1
         const double* Vk = vp[k];
@@ -743,7 +820,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         hb.scale = d_nrm + k;
         HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
         for (int p = 0; p < P; ++p) {
-            if (dim[p] <= kSmall || done_h[p]) continue;
+            if (dim[p] <= kSmall || h->pinned_i32[p]) continue;  // flags as of the last check
             int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
             if (rc) return rc;
         }
@@ -752,48 +829,61 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 2, 0.5, 1e-14, st));  // again where DGKS asks
         const int m = k + 1;
         if (m < 20 || m % 5 == 0 || m == M) {
-            HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(mfin_h.data(), d_mfinal, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(al_h.data(), d_alpha, al_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(be_h.data(), d_beta, be_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            // convergence of the lowest nev Ritz pairs, every active problem in parallel
-            std::vector<int> conv(P, 0), qlbad(P, 0);
-            parallel_for(P, [&](int p) {
-                if (done_h[p] || m < nev) return;
-                std::vector<double> d(al_h.begin() + (size_t)p * ldh, al_h.begin() + (size_t)p * ldh + m);
-                std::vector<double> ee(be_h.begin() + (size_t)p * ldh, be_h.begin() + (size_t)p * ldh + (m - 1));
-                std::vector<double> Z((size_t)m, 0.0);
-                Z[m - 1] = 1.0;
-                if (!tridiag_ql(d, ee, 1, &Z)) {
-                    qlbad[p] = 1;
-                    return;
+            if (P > 1) {
+                // several problems: the convergence test of every running problem on the device (one
+                // launch); the host reads back only the done flags (pinned memory, one synchronisation)
+                HIPCHK(launch_eig_conv(e, m, nev, h->gs_tol, d_zs, st));
+                HIPCHK(hipMemcpyAsync(h->pinned_i32, d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                active = 0;
+                for (int p = 0; p < P; ++p) active += h->pinned_i32[p] == 0;
+            } else {
+                // one problem: its tridiagonal QL on the host (microseconds) beats one wave on the device
+                HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(mfin_h.data(), d_mfinal, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(al_h.data(), d_alpha, al_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(be_h.data(), d_beta, be_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                // convergence of the lowest nev Ritz pairs, every active problem in parallel
+                std::vector<int> conv(P, 0), qlbad(P, 0);
+                parallel_for(P, [&](int p) {
+                    if (done_h[p] || m < nev) return;
+                    std::vector<double> d(al_h.begin() + (size_t)p * ldh, al_h.begin() + (size_t)p * ldh + m);
+                    std::vector<double> ee(be_h.begin() + (size_t)p * ldh, be_h.begin() + (size_t)p * ldh + (m - 1));
+                    std::vector<double> Z((size_t)m, 0.0);
+                    Z[m - 1] = 1.0;
+                    if (!tridiag_ql(d, ee, 1, &Z)) {
+                        qlbad[p] = 1;
+                        return;
+                    }
+                    const double bk = be_h[(size_t)p * ldh + m - 1];
+                    bool ok = true;
+                    for (int i = 0; i < std::min(nev, m) && ok; ++i)
+                        if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) ok = false;
+                    conv[p] = ok;
+                });
+                bool changed = false;
+                active = 0;
+                for (int p = 0; p < P; ++p) {
+                    if (qlbad[p]) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
+                    if (done_h[p]) continue;
+                    if (conv[p]) {
+                        done_h[p] = 1;
+                        mfin_h[p] = m;
+                        changed = true;
+                        continue;
+                    }
+                    ++active;
                 }
-                const double bk = be_h[(size_t)p * ldh + m - 1];
-                bool ok = true;
-                for (int i = 0; i < std::min(nev, m) && ok; ++i)
-                    if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) ok = false;
-                conv[p] = ok;
-            });
-            bool changed = false;
-            active = 0;
-            for (int p = 0; p < P; ++p) {
-                if (qlbad[p]) return fail(h, MOED_ECONV, "tridiagonal QL did not converge");
-                if (done_h[p]) continue;
-                if (conv[p]) {
-                    done_h[p] = 1;
-                    mfin_h[p] = m;
-                    changed = true;
-                    continue;
+                if (changed) {
+                    HIPCHK(hipMemcpyAsync(d_done, done_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+                    HIPCHK(hipMemcpyAsync(d_mfinal, mfin_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
                 }
-                ++active;
-            }
-            if (changed) {
-                HIPCHK(hipMemcpyAsync(d_done, done_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
-                HIPCHK(hipMemcpyAsync(d_mfinal, mfin_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+                h->pinned_i32[0] = done_h[0];
             }
         }
     }
+    if (!pqueue.empty()) HIPCHK(hipStreamWaitEvent(st, h->ev_join, 0));
     // final Ritz pairs: T_m of every problem on the host (in parallel), S = Z / |u_k|
     HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(mfin_h.data(), d_mfinal, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1300,16 +1390,18 @@ int moed_create(const moed_params* p, moed_handle** out) {
     interaction_constants(h->U, h->J, &h->UmJ3, &h->Um2J);
     hipError_t e = hipSetDevice(h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device);
-    if (e != hipSuccess) {
+    if (e == hipSuccess && fit_scratch_address(h->device, &h->fit_scratch) != MOED_OK) e = hipErrorInvalidValue;
+    if (e != hipSuccess) {  // (device is current: fit_scratch_address looks up this GPU's array)
+        if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+        if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+        if (h->stream2) (void)hipStreamDestroy(h->stream2);
         if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
-    }
-    if (fit_scratch_address(h->device, &h->fit_scratch) != MOED_OK) {  // device is current
-        (void)hipStreamDestroy(h->stream);
-        delete h;
-        return MOED_EHIP;
     }
     int rc = set_model(h, p->ek, p->vk);
     if (rc) {
@@ -1328,8 +1420,13 @@ int moed_destroy(moed_handle* h) {
     h->eig.clear();
     h->basis.clear();
     for (auto& w : h->ws) w.release();
+    if (h->pinned_i32) (void)hipHostFree(h->pinned_i32);
     h->d_ek.release();
     h->d_coef.release();
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return MOED_OK;
@@ -1466,7 +1563,7 @@ int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int
                 bout.push_back(h->eig[is].get());
             }
         }
-        int rc = eig_batch(h, big, nev, bout);
+        int rc = eig_batch(h, big, nev, bout, true);
         if (rc) return rc;
     } else {
         for (int is = 0; is < nsec; ++is) {
@@ -2024,6 +2121,14 @@ int moed_fit_nfev(moed_handle* h, int32_t nfit, int32_t* nfev) {
     if (!h || !nfev || nfit < 0) return MOED_EARG;
     if ((size_t)nfit > h->fit_nfev.size()) return fail(h, MOED_ESTATE, "fit_nfev: fewer fits in the last moed_minimize");
     std::copy(h->fit_nfev.begin(), h->fit_nfev.begin() + nfit, nfev);
+    return MOED_OK;
+}
+
+int moed_set_persistent(moed_handle* h, int64_t max_dim) {
+    if (!h) return MOED_EARG;
+    if (max_dim < -1 || max_dim > kPersistMaxDim)
+        return fail(h, MOED_EARG, "max_dim must be -1 (default), 0 (off) or at most 2^20");
+    h->persist_max = max_dim;
     return MOED_OK;
 }
 

@@ -155,9 +155,40 @@ hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b,
 //   pass 2: (problems flagged) pass 1 again; after pass 1 or 2 the step's alpha, beta and stop test
 hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1, const double* w, int k, int pass,
                            double eta2, double tiny, hipStream_t s);
+// convergence after a step with basis size m, on the device: for every running problem, the lowest
+// nev Ritz pairs of T_m pass |beta_m z_{m,i}| <= tol |theta_i| -> done = 1, mfinal = m
+// (zs: [P][nev][3 ldh] workspace)
+hipError_t launch_eig_conv(const EigArgs& e, int m, int nev, double tol, double* zs, hipStream_t s);
 // Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
 hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s);
+
+// Persistent work-queue eigensolver (moed_persist.hip): one launch runs the whole Lanczos iteration
+// with partial reorthogonalisation and the convergence test of every queued small problem, one
+// workgroup per problem at a time, pulling problems from *head (0 at launch) in `order`.
+constexpr int kPersistThreads = 1024;
+constexpr int kPersistMaxBasis = 402;  // ldh cap: 400 + 2
+constexpr int kPersistMaxU = 1024;     // up strings per sector row
+constexpr int64_t kPersistMaxDim = (int64_t)1 << 20;
+constexpr int64_t kPersistDefaultDim = 8192;  // default size limit of the persistent launch
+struct PersistArgs {
+    EigArgs e;                  // alpha, beta, nrm, om, done, mfinal, mmax, pdim, ldh (zeroed tables)
+    const BatchProbDev* prob;
+    double* const* V;           // addresses of the basis vectors 0..mmax (packed like w)
+    double* W;                  // packed work vector
+    const double* ek;
+    const double* slotc;
+    double U, UmJ3, Um2J, J;
+    const uint64_t* seed;       // [P] start-vector seeds
+    const int32_t* order;       // [nq] queued problems, largest first
+    int32_t nq;
+    int32_t* head;              // queue head
+    int32_t nev;
+    int32_t max_nU;             // largest up-string count of the queued problems
+    double tol;                 // |beta_m z_{m,i}| <= tol |theta_i|
+    double* zs;                 // [P][nev][3 ldh] inverse-iteration workspace
+};
+hipError_t launch_eig_persistent(const PersistArgs& a, int grid, int norb, int nslot, hipStream_t s);
 
 // many reference recurrences at once (the Green's-function chains), packed like the eigensolver's
 // problems; A, B, stop, nsc per chain (A, B with row length ldh)

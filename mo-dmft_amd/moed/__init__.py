@@ -61,7 +61,7 @@ EXPORTS = [
     "moed_fill_start_device", "moed_set_hv_shape", "moed_set_eig_chunk", "moed_stream", "moed_band_tb",
     "moed_delta_new", "moed_set_fit_launch", "moed_fit_nfev", "moed_dos_diag", "moed_dos", "moed_eigvec_filename",
     "moed_export_eigvec", "moed_import_eigvec", "moed_export_green", "moed_import_green",
-    "moed_minimize_device_noalloc",
+    "moed_minimize_device_noalloc", "moed_set_persistent",
 ]
 
 _lib = None
@@ -106,6 +106,7 @@ def lib():
         L.moed_fill_start_device.argtypes = [ct.c_void_p, ct.c_uint64, ct.c_int64, ct.c_void_p, ct.c_void_p]
         L.moed_set_hv_shape.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
         L.moed_set_eig_chunk.argtypes = [ct.c_void_p, ct.c_int32]
+        L.moed_set_persistent.argtypes = [ct.c_void_p, ct.c_int64]
         L.moed_set_fit_launch.argtypes = [ct.c_void_p, ct.c_int32]
         L.moed_dos_diag.argtypes = [ct.c_void_p, ct.c_int32, _dp, ct.c_double, ct.c_double, ct.c_int32, ct.c_int32,
                                     _dp, _dp, _dp, _dp, _dp]
@@ -434,6 +435,11 @@ class EDSolver:
 
     def set_eig_chunk(self, vectors=0):
         self._check(lib().moed_set_eig_chunk(self.h, vectors))
+
+    def set_persistent(self, max_dim=-1):
+        """largest sector that the batched solve runs in its persistent work-queue launch
+        (-1: default, 0: none)"""
+        self._check(lib().moed_set_persistent(self.h, int(max_dim)))
 
     def set_fit_launch(self, iterations=0):
         """CG iterations per fit and kernel launch inside minimization (0: default)"""

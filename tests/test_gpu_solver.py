@@ -176,9 +176,9 @@ def test_four_orbitals_with_tij_against_dense_ed():
 
 
 def test_batched_solve_and_green_equal_sector_by_sector():
-    """7-site model, all 64 sectors (dims up to 1225): the multi-problem run gives the same
-    eigenvalues and kept states as the sector-by-sector run, bit for bit (every problem follows the
-    same arithmetic alone or in the batch), and G(iw) within 1e-12"""
+    """7-site model, all 64 sectors (dims up to 1225): the multi-problem run (persistent work-queue
+    kernel) gives the same kept states as the sector-by-sector run, eigenvalues within 1e-12 and
+    G(iw) within 1e-12"""
     rng = np.random.default_rng(77)
     norb, nbath = 2, 5
     ns = norb + nbath
@@ -196,8 +196,14 @@ def test_batched_solve_and_green_equal_sector_by_sector():
             evs = np.concatenate([s.sector_eigenvalues(i + 1) for i in range(len(sectors))])
             out.append((kept, evs, s.ed_calc_green_ftn(om)))
     (k0, e0, g0), (k1, e1, g1) = out
-    assert k0 == k1
-    assert np.array_equal(e0, e1)
+    # the batched solve runs the small sectors in the persistent work-queue kernel (its own
+    # reductions, so rounded differently); the same states are kept
+    assert [k[2:] for k in k0] == [k[2:] for k in k1]
+    fin = e0 < np.finfo(float).max
+    assert np.array_equal(fin, e1 < np.finfo(float).max)
+    assert np.max(np.abs(e0[fin] - e1[fin])) <= 1e-12 * np.max(np.abs(e0[fin]))
+    assert np.allclose([k[0] for k in k0], [k[0] for k in k1], rtol=1e-12, atol=0)
+    assert np.allclose([k[1] for k in k0], [k[1] for k in k1], rtol=1e-10, atol=1e-12)
     # the batched Green's-function chains reduce per 1024-element item (rounded differently from
     # the chain-by-chain sums, deterministic either way)
     assert np.max(np.abs(g0 - g1)) <= 1e-12 * np.max(np.abs(g0))
