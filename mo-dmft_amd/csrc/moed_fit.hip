@@ -81,7 +81,13 @@ struct FitSmem {
     union {
         double fstage[2][kFitChunkF * kMaxOrb];  // FUNC terms, two chunks (reference order)
         double cols[kFitCols * kFitChunkG];      // gradient terms of one chunk (reference order)
+        struct {  // tree order, Nw <= kFitThreads: thread t's frequency iw = t, set once per launch
+            double zi[kFitThreads], rw[kFitThreads];                  // Im z, 1/w_n
+            double dre[kMaxOrb][kFitThreads], dim[kMaxOrb][kFitThreads];  // the target Dev_o(w)
+        } tc;
     } u;
+    double fred[kFitThreads / 64];  // tree-order FUNC: the waves' partial sums
+    int32_t tc_ok;                  // u.tc holds the frequencies of this launch
 };
 
 // the fit's LDS, shared by every device function of this file (one workgroup = one fit)
@@ -330,7 +336,86 @@ __device__ __noinline__ double eval_func_ref(const FitProb qa) {
     return S.bcast[0];
 }
 
+// Tree order with the frequencies cached in LDS (Nw <= kFitThreads): thread t's frequency iw = t.
+// Fill once per launch; ends with a barrier.
+__device__ void tree_cache_fill(const FitProb qa) {
+    const FitView q = view(qa);
+    const int t = threadIdx.x;
+    const bool on = uniform_i32(qa.tree) && q.nw <= kFitThreads;
+    if (on) {
+        const bool ok = t < q.nw;
+        const double w = ok ? q.omega[t] : 1.0;
+        S.u.tc.zi[t] = zim(q, w);
+        S.u.tc.rw[t] = ok ? 1.0 / w : 0.0;
+        for (int o = 0; o < q.norb; ++o) {
+            const size_t io = (size_t)o + (size_t)q.norb * (ok ? t : 0);
+            S.u.tc.dre[o][t] = q.dev[2 * io];
+            S.u.tc.dim[o][t] = q.dev[2 * io + 1];
+        }
+    }
+    if (t == 0) S.tc_ok = on;
+    __syncthreads();
+}
+
+// the nonzero-coupling lists of nonzero_couplings for wave 0 only, without the barrier: the caller
+// has wave 0 write xt and then call this (LDS operations of one wave complete in order)
+__device__ __forceinline__ void nonzero_couplings_wave0(const FitView& q) {
+    const int tid = threadIdx.x;
+    const int o = tid >> 4, k = tid & 15;
+    const bool in = o < q.norb && k < q.nbath;
+    const double v = in ? S.xt[q.nbath + o * q.nbath + k] : 0.0;
+    const bool nz = in && v != 0.0;
+    const uint64_t mask = __ballot(nz);
+    const uint64_t seg = mask & (0xffffull << (16 * o));
+    if (nz) {
+        const int n = __popcll(seg & ((1ull << tid) - 1ull));
+        S.nzv2[o][n] = v * v;
+        S.nzv[o][n] = v;
+        S.nzek[o][n] = S.xt[k];
+        S.nzk[o][n] = k;
+    }
+    if (k == 0 && o < q.norb) S.nnz[o] = __popcll(seg);
+}
+
+// FUNC in tree order from the cached frequencies and the coupling lists already built for xt;
+// the same operations as eval_func_tree with one frequency per thread, one barrier: every thread
+// adds the waves' partials in index order (block_sum's order).  S.fred is written again only by
+// the next evaluation, after the barrier of its point update.
+__device__ __forceinline__ double eval_func_tree_cached(const FitProb& qa) {
+#pragma clang fp contract(off)
+    const FitView q = view(qa);
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    if (tid < q.nw) {
+        const double zi = S.u.tc.zi[tid], rw = S.u.tc.rw[tid];
+        for (int o = 0; o < q.norb; ++o) {
+            double re = 0.0, im = 0.0;
+            const int nn = uniform_i32(S.nnz[o]);
+            for (int n = 0; n < nn; ++n) {
+                const cxd d = inv_den(S.nzek[o][n], zi);
+                const double v2 = S.nzv2[o][n];
+                re = fma(v2, d.re, re);
+                im = fma(v2, d.im, im);
+            }
+            re = (re + S.xt[q.nx - q.norb + o]) - S.u.tc.dre[o][tid];
+            im = im - S.u.tc.dim[o][tid];
+            s = fma(fma(re, re, im * im), rw, s);
+        }
+    }
+    s = wave_sum(s);
+    if ((tid & 63) == 0) S.fred[tid >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < kFitThreads / 64; ++w) tot += S.fred[w];
+    return tot / (double)(float)(q.norb * q.norb * q.nw);
+}
+
 __device__ __forceinline__ double eval_func(const FitProb& q) {
+    if (S.tc_ok) {  // block-uniform
+        if (threadIdx.x < 64) nonzero_couplings_wave0(view(q));
+        __syncthreads();
+        return eval_func_tree_cached(q);
+    }
     return uniform_i32(q.tree) ? eval_func_tree(q) : eval_func_ref(q);
 }
 
@@ -474,9 +559,21 @@ __device__ __forceinline__ void eval_grad(const FitProb& q, double* out) {
 // f1dim (frprmn.f90:212-234)
 __device__ __forceinline__ double f1dim(const FitProb& q, double xv, int& nfev) {
 #pragma clang fp contract(off)
+    ++nfev;
+    if (S.tc_ok) {  // nx <= 64: wave 0 writes the point and builds its coupling lists, one barrier
+        if (threadIdx.x < 64) {
+            const int j = threadIdx.x;
+            if (j < q.nx) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            nonzero_couplings_wave0(view(q));
+        }
+        __syncthreads();
+        return eval_func_tree_cached(q);
+    }
     for (int j = threadIdx.x; j < q.nx; j += kFitThreads) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
     __syncthreads();
-    ++nfev;
     return eval_func(q);
 }
 
@@ -637,6 +734,7 @@ __global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
     double* px = a.x + (size_t)f * n;
     const int phase = uniform_i32(ist[0]);
     if (phase == kPhaseDone) return;
+    tree_cache_fill(q);
     const double EPS = 1.e-10, LTOL = (double)1.e-8f;  // frprmn.f90:12, :91
     int its0 = uniform_i32(ist[1]), nfev = uniform_i32(ist[3]);
     double fp = st[3 * kFitNx], fret = st[3 * kFitNx + 1];
@@ -738,6 +836,7 @@ __global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
 // FUNC and/or dfunc at x[blockIdx.x]
 __global__ void __launch_bounds__(kFitThreads) fit_eval_kernel(FitEvalArgs a) {
     const int f = blockIdx.x, n = a.q.nx;
+    tree_cache_fill(a.q);
     for (int j = threadIdx.x; j < n; j += kFitThreads) S.xt[j] = a.x[(size_t)f * n + j];
     __syncthreads();
     if (a.f) {

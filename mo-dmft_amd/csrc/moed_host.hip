@@ -141,6 +141,23 @@ struct moed_handle {
     DevBuf ws[9];
     DevBuf eng[8];  // multi-problem Lanczos engine (eig_batch)
     DevBuf gbuf[5];  // batched Green's-function chains (green_batch)
+    // instantiated HIP graphs of the reference recurrence's steps 1..nstep (lanczos_ref_dev), per
+    // sector, step count and buffer set; cleared whenever a captured pointer may change
+    struct LzGraph {
+        int nu, nd, nstep;
+        int64_t ldx;
+        const void* ptr[6];
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<LzGraph> lz_graphs;
+    void clear_graphs() {
+        for (auto& g : lz_graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        lz_graphs.clear();
+    }
     std::vector<std::unique_ptr<DevBuf>> basis;  // its Krylov basis, in chunks of whole vectors
 };
 
@@ -224,6 +241,7 @@ int set_model(moed_handle* h, const double* ek, const double* vk) {
     for (size_t k = 0; k < h->slots.size(); ++k) slotc[k] = h->slots[k].c;
     HIPCHK(h->d_slotc.ensure(slotc.size() * sizeof(double)));
     HIPCHK(hipMemcpy(h->d_slotc.p, slotc.data(), slotc.size() * sizeof(double), hipMemcpyHostToDevice));
+    h->clear_graphs();  // captured launches hold table and buffer addresses
     h->tables.clear();  // onsite sums and move lists depend on ek, vk
     h->solved = false;
     return MOED_OK;
@@ -384,6 +402,27 @@ int hv_groups(moed_handle* h, const Sector& s, int64_t ldx = 0) { return hv_plan
 
 // y = H x (x != y) on stream st; optional fused y *= 1/yscale[0] and x.y partials.  ldx: row stride
 // of x and y (0: nU); padding columns of x must hold zeros and are never written in y.
+HvArgs hv_args(moed_handle* h, const Sector& s, const double* x, double* y, double* dot_part, const double* yscale,
+               const int32_t* stop, int64_t ldx) {
+    HvArgs a{};
+    a.x = x;
+    a.y = y;
+    a.ldx = ldx > 0 ? ldx : s.up->count;
+    a.up = s.up->dev();
+    a.dn = s.dn->dev();
+    a.ek = h->d_ek.as<double>();
+    a.coef = h->d_coef.as<double>();
+    a.slotc = h->d_slotc.as<double>();
+    a.U = h->U;
+    a.UmJ3 = h->UmJ3;
+    a.Um2J = h->Um2J;
+    a.J = h->J;
+    a.dot_part = dot_part;
+    a.yscale = yscale;
+    a.stop = stop;
+    return a;
+}
+
 int hv_dev(moed_handle* h, const Sector& s, const double* x, double* y, double* dot_part, const double* yscale,
            const int32_t* stop, hipStream_t st, int64_t ldx = 0) {
     HvArgs a{};
@@ -495,13 +534,39 @@ int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep
     int rc = hv_dev(h, s, u[0], w, p, nullptr, sc.stop, st, ldx);
     if (rc) return rc;
     HIPCHK(launch_lz_finish_a(0, p, ngrp, sc, kBreakdown, st));
-    for (int j = 1; j <= nstep; ++j) {
-        HIPCHK(launch_lz_update(j, w, u[(j - 1) % 3], u[(j + 1) % 3], u[j % 3], D, sc, p, st));
-        HIPCHK(launch_lz_finish_b(j, p, kRedBlocks, sc, st));
-        rc = hv_dev(h, s, u[j % 3], w, p, nullptr, sc.stop, st, ldx);
-        if (rc) return rc;
-        HIPCHK(launch_lz_finish_a(j, p, ngrp, sc, kBreakdown, st));
+    // steps 1..nstep: a fixed sequence of kernel launches (nothing else), captured once per sector,
+    // step count and buffer set into a HIP graph and replayed as one launch on later calls
+    const void* key_ptr[6] = {V.p, W.p, part.p, scal.p, s.up->d_rec.p, s.dn->d_mv.p};
+    moed_handle::LzGraph* gr = nullptr;
+    for (auto& g : h->lz_graphs)
+        if (g.nu == s.nu && g.nd == s.nd && g.nstep == nstep && g.ldx == ldx &&
+            std::equal(key_ptr, key_ptr + 6, g.ptr))
+            gr = &g;
+    if (!gr) {
+        moed_handle::LzGraph g{s.nu, s.nd, nstep, ldx, {}, nullptr, nullptr};
+        std::copy(key_ptr, key_ptr + 6, g.ptr);
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        for (int j = 1; j <= nstep && ce == hipSuccess; ++j) {
+            ce = launch_lz_update(j, w, u[(j - 1) % 3], u[(j + 1) % 3], u[j % 3], D, sc, p, st);
+            if (ce == hipSuccess) ce = launch_lz_finish_b(j, p, kRedBlocks, sc, st);
+            if (ce == hipSuccess) {
+                HvArgs a = hv_args(h, s, u[j % 3], w, p, nullptr, sc.stop, ldx);
+                ce = launch_hv(a, hv_plan_of(h, s, ldx), h->norb, st);
+            }
+            if (ce == hipSuccess) ce = launch_lz_finish_a(j, p, ngrp, sc, kBreakdown, st);
+        }
+        const hipError_t ee = hipStreamEndCapture(st, &g.graph);  // always ends the capture
+        if (ce == hipSuccess) ce = ee;
+        if (ce == hipSuccess) ce = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+            return hipfail(h, ce, "capturing the Lanczos steps in a HIP graph");
+        }
+        h->lz_graphs.push_back(g);
+        gr = &h->lz_graphs.back();
     }
+    HIPCHK(hipGraphLaunch(gr->exec, st));
     std::vector<double> hs(2 * (nstep + 1) + 2);
     HIPCHK(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1416,6 +1481,7 @@ int moed_destroy(moed_handle* h) {
     if (!h) return MOED_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->clear_graphs();
     h->tables.clear();
     h->eig.clear();
     h->basis.clear();
@@ -2139,6 +2205,7 @@ int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip) {
     if (strip != 0 && strip != 64 && strip != 128) return fail(h, MOED_EARG, "strip must be 0, 64 or 128");
     h->hv_threads = threads;
     h->hv_strip = strip;
+    h->clear_graphs();
     return MOED_OK;
 }
 

@@ -467,3 +467,25 @@ def test_green_chains_batched_match_sequential(beta):
         for key, n in (("ap", a["nsp"]), ("bp", a["nsp"]), ("an", a["nsn"]), ("bn", a["nsn"])):
             m = min(n, 8)
             assert np.allclose(a[key][:m], b[key][:m], rtol=1e-10, atol=1e-12), key
+
+
+def test_lanczos_graph_replays_are_bit_identical(configs):
+    """lanczos_iteration's steps run as a HIP graph captured on the first call for a sector and
+    replayed afterwards: calls on two sectors, interleaved, and a fresh handle give the same a, b
+    bit for bit"""
+    c = configs["ns8_plain"]
+    out = {}
+    with EDSolver.from_config(c) as s:
+        for (nu, nd) in ((4, 4), (5, 4), (4, 4), (5, 4), (4, 4)):
+            x = start_vector(99 + nu, s.dim(nu, nd))
+            out.setdefault((nu, nd), []).append(s.lanczos_iteration(nu, nd, x, 60))
+    with EDSolver.from_config(c) as s:
+        fresh = s.lanczos_iteration(5, 4, start_vector(104, s.dim(5, 4)), 60)
+    for runs in out.values():
+        for r in runs[1:]:
+            assert r[0] == runs[0][0] and np.array_equal(r[1], runs[0][1]) and np.array_equal(r[2], runs[0][2])
+    assert fresh[0] == out[(5, 4)][0][0] and np.array_equal(fresh[1], out[(5, 4)][0][1])
+    m = omodel(c)
+    b54 = oracle.Basis(c.nsite, 5, 4)
+    nsco, ao, bo = oracle.lanczos(m, b54, start_vector(104, b54.ntot), 60)
+    assert np.allclose(out[(5, 4)][0][1][:20], ao[:20], rtol=1e-10, atol=1e-12)
