@@ -213,10 +213,16 @@ class EDSolver:
         self.h = h
 
     @classmethod
-    def from_config(cls, cfg, **kw):
+    def from_config(cls, cfg, tuned=True, **kw):
+        """a solver for a named workload; with tuned, the launch shapes the autotuner picked for it
+        (moed/tuning.json: results are bit-identical for every shape, only the speed differs)"""
         args = dict(beta=cfg.beta, nstep=cfg.nstep, nev=cfg.nev, nw=cfg.nw, sectors=cfg.sectors, tij=cfg.tij)
         args.update(kw)
-        return cls(cfg.norb, cfg.nbath, cfg.U, cfg.J, cfg.ek, cfg.vk, **args)
+        s = cls(cfg.norb, cfg.nbath, cfg.U, cfg.J, cfg.ek, cfg.vk, **args)
+        if tuned:
+            from .autotune import apply
+            apply(s, cfg.name)
+        return s
 
     def close(self):
         if getattr(self, "h", None):

@@ -74,3 +74,18 @@ def test_import_eigvec_rejects_missing_and_malformed_files(tmp_path):
     (tmp_path / str(z["eig_name"])).write_bytes(z["eig_bytes"].tobytes()[:-5])
     with pytest.raises(Exception):
         moed.import_eigvec(str(tmp_path), 7, 3)
+
+
+def test_tuning_table_names_workloads_and_valid_shapes():
+    """the autotuner's committed table (moed/tuning.json): every entry names a workload of
+    tests/golden/configs.json and shapes that moed_set_hv_shape / moed_set_persistent accept"""
+    from moed import autotune
+    from moed.model import load_configs
+    names = {c.name for c in load_configs()}
+    table = autotune.load_table()
+    assert table and set(table) <= names
+    for w, t in table.items():
+        assert (t["hv_threads"], t["hv_strip"]) in autotune.HV_SHAPES
+        assert t["hv_us"][f"{t['hv_threads']}x{t['hv_strip']}"] == min(t["hv_us"].values())
+        if "persistent_max_dim" in t:
+            assert t["persistent_max_dim"] in autotune.PERSIST
