@@ -66,6 +66,123 @@ def cpu_reference_hv(cfg, nu, nd, calls=2):
                       f"{sec:.3f} s per H v", "seconds_per_hv": sec}
 
 
+def _ref_run(task, cfg, nu, nd, i1, payload=(), ranks=None, timeout=300):
+    """run one task of the reference build over MPI ranks on this host; raw output bytes or None"""
+    import numpy as np
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(drv) and os.path.exists(mpiexec)):
+        return None
+    n = ranks or min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<8i", task, cfg.norb, cfg.nbath, nu, nd, i1, 0, 0))
+            f.write(struct.pack("<2d", cfg.U, cfg.J))
+            f.write(np.asarray(cfg.ek, "<f8").tobytes())
+            f.write(np.asarray(cfg.vk, "<f8").reshape(-1, order="F").tobytes())
+            for p in payload:
+                f.write(np.ascontiguousarray(p).tobytes())
+        r = subprocess.run([mpiexec, "-n", str(n), drv, fin, fout], capture_output=True, text=True,
+                           timeout=timeout, cwd=td)
+        if r.returncode != 0 or not os.path.exists(fout):
+            return None
+        return open(fout, "rb").read()
+
+
+def cpu_extra_baselines(gpu):
+    """CPU baselines beyond H v, each beside the GPU time of the same work:
+    lanczos_iteration steps at the 14-site sector and one minimization at the 15-site fit shape
+    (reference build, MPI ranks on the host cores), and ground-state solves with the oracle's C
+    restatement of multiply_H under SciPy's ARPACK (the reference's diag calls PARPACK, absent here)"""
+    import numpy as np
+    out = {}
+    c3 = config_of("ns14_roofline")
+    nu, nd = c3.sectors[0]
+    raw = _ref_run(14, c3, nu, nd, 4)
+    if raw:
+        sec = struct.unpack("<d", raw[:8])[0]
+        out["lanczos_step_ns14"] = {
+            "kind": "reference", "cores": struct.unpack("<i", raw[8:12])[0], "cpu_s_per_step": round(sec, 4),
+            "gpu_s_per_step": gpu.get("lanczos_step_ns14"),
+            "sample": "lanczos_iteration (ed_lanczos.F90:14-68) with Nstep = 4 on the 14-site sector (5 products), "
+                      "rows split over the MPI ranks"}
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ns15_fit.npz"))
+    c5 = config_of("ns15_fit")
+    dflat = np.stack([z["Dev"].T.reshape(-1).real, z["Dev"].T.reshape(-1).imag], 1).reshape(-1)
+    raw = _ref_run(15, c5, 0, 0, len(z["omega"]), [z["x0"], z["omega"], dflat])
+    if raw:
+        sec = struct.unpack("<d", raw[:8])[0]
+        out["minimization_ns15_shape"] = {
+            "kind": "reference", "cores": struct.unpack("<i", raw[8:12])[0], "cpu_s_one_fit": round(sec, 3),
+            "cpu_s_256_fits_extrapolated": round(256 * sec, 1), "gpu_s_one_fit": gpu.get("fit_one_ns15"),
+            "gpu_s_256_fits": gpu.get("fit_256_ns15"),
+            "sample": "one minimization (func.f90:144 -> frprmn) at nx 51, 4096 frequencies (tests/golden/"
+                      "ns15_fit.npz), frequencies split over the MPI ranks as ed_green_init splits them; the "
+                      "256-fit figure is that time x 256 (extrapolated, not run)"}
+    import oracle
+    import scipy.sparse.linalg as sla
+    from moed.model import start_vector
+    nt = min(16, os.cpu_count() or 1)
+    for name in ("ns8_plain", "ns12_plain"):
+        c = config_of(name)
+        su, sd = c.sectors[0]
+        m = oracle.Model(c.norb, c.nbath, c.U, c.J, c.ek, c.vk, c.tij)
+        b = oracle.Basis(c.nsite, su, sd)
+        op = sla.LinearOperator((b.ntot, b.ntot), matvec=lambda v: oracle.multiply_H(m, b, v, nt), dtype=np.float64)
+        t = time.perf_counter()
+        w = sla.eigsh(op, k=1, which="SA", tol=1e-12, v0=start_vector(4242, b.ntot), return_eigenvectors=False)
+        sec = time.perf_counter() - t
+        out[f"ground_state_{name}"] = {
+            "kind": "port", "cores": nt, "cpu_s": round(sec, 3), "gpu_s": gpu.get(f"ground_state_{name}"),
+            "E0_cpu": float(w[0]), "E0_gpu": gpu.get(f"E0_{name}"),
+            "sample": f"sector {su, sd} (D = {b.ntot}): SciPy eigsh (ARPACK dsaupd, which=SA, tol 1e-12) on "
+                      f"the oracle's C restatement of multiply_H with {nt} OpenMP threads"}
+    return out
+
+
+def gpu_extra_timings():
+    """GPU times of the work the extra CPU baselines time"""
+    import numpy as np
+    import moed
+    from moed.model import config, start_vector
+    out = {}
+    c3 = config("ns14_roofline")
+    nu, nd = c3.sectors[0]
+    with moed.EDSolver.from_config(c3) as s:
+        x = start_vector(5, s.dim(nu, nd))
+        s.lanczos_iteration(nu, nd, x, 4)  # captures the step graph
+        t = time.perf_counter()
+        s.lanczos_iteration(nu, nd, x, 200)
+        out["lanczos_step_ns14"] = round((time.perf_counter() - t) / 201, 6)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ns15_fit.npz"))
+    norb, nbath = int(z["norb"]), int(z["nbath"])
+    with moed.EDSolver(norb, nbath, 0.0, 0.0, np.zeros(norb + nbath), np.zeros((norb, nbath))) as s:
+        t = time.perf_counter()
+        s.minimization(z["x0"], z["omega"], z["Dev"])
+        out["fit_one_ns15"] = round(time.perf_counter() - t, 3)
+        rng = np.random.default_rng(11)
+        X = z["x0"][None, :] * (1 + 0.1 * rng.uniform(-1, 1, (256, len(z["x0"]))))
+        t = time.perf_counter()
+        s.minimization(X, z["omega"], z["Dev"])
+        out["fit_256_ns15"] = round(time.perf_counter() - t, 3)
+    for name in ("ns8_plain", "ns12_plain"):
+        c = config(name)
+        with moed.EDSolver.from_config(c) as s:
+            s.ed_solve()  # tables
+        with moed.EDSolver.from_config(c) as s:
+            t = time.perf_counter()
+            kept = s.ed_solve()
+            out[f"ground_state_{name}"] = round(time.perf_counter() - t, 4)
+            out[f"E0_{name}"] = kept[0][0]
+    return out
+
+
+def config_of(name):
+    from moed.model import config
+    return config(name)
+
+
 def cpu_port_hv(cfg, nu, nd):
     """fallback: the oracle's C restatement with OpenMP over rows"""
     import numpy as np
@@ -316,9 +433,10 @@ def main():
     gbs = alg_bytes / (ms * 1e-3) / 1e9
     traffic = hbm_traffic(os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.md"))
     batched = None if args.no_batched else batched_vs_plain()
-    cpu = None
+    cpu = cpu_extra = None
     if not args.no_cpu:
         cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
+        cpu_extra = cpu_extra_baselines(gpu_extra_timings())
     out = {
         "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 6), "higher_is_better": True, "scaling": "weak",
@@ -331,6 +449,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes,
                      "kernel": "one H v = hv_rows_kernel + hv_strip_kernel on one stream"},
         "cpu_baseline": cpu,
+        "cpu_baseline_more": cpu_extra,
         "batched_vs_plain": batched,
         "wall_s_warm_and_timed": round(wall, 4),
         "untimed_load_steps": n_warm * args.steps,

@@ -32,6 +32,10 @@
 !  13 dos (dos.f90:1-63) reading such records: i1 = number of records, i2 = nw, i3 = nev_calc;
 !                 payload as task 12, then real64 (small, Nstep), omega_real(nw), and (val, prob) of
 !                 the nev_calc states -> Aff(norb, nw); Nstep bounds every record's step counts
+!  14 timing of lanczos_iteration, Nstep = i1, rows split over the MPI ranks -> seconds per step,
+!                 ranks, a(0)
+!  15 timing of one minimization with the frequencies split over the ranks as ed_green_init splits
+!                 them; payload as task 5 -> seconds, ranks, fret, x(nx)
 program ref_driver
     use parallel_params
     use ed_config
@@ -227,6 +231,60 @@ program ref_driver
         enddo
         call dos(om, aff)
         write(11) aff
+    case (14)
+        ! time lanczos_iteration with Nstep = i1 (i1 + 1 products) on the sector; every rank owns
+        ! nloc rows, as in the reference run -> seconds per Lanczos step (per product), ranks, a(0)
+        Nstep = i1
+        bas = generate_basis(ne_up, ne_dn)
+        allocate(x(bas%nloc), a(0:nstep), b(0:nstep))
+        do i = 1, bas%nloc
+            x(i) = 1.0d0 / dble(i + bas%offsets(node))
+        enddo
+        call mpi_barrier(comm, ierr)
+        t0 = mpi_wtime()
+        call lanczos_iteration(bas, x, nstep_calc, a, b)
+        call mpi_barrier(comm, ierr)
+        t1 = mpi_wtime()
+        if (node .eq. 0) then
+            write(11) (t1 - t0) / dble(nstep + 1)
+            write(11) nodes
+            write(11) a(0)
+        endif
+    case (15)
+        ! time one minimization (func.f90:144) with the frequencies split over the ranks as
+        ! ed_green_init splits them (ed_green.F90:29-51); payload as task 5 (all nw frequencies)
+        ! -> seconds, ranks, fret, x
+        nwl = i1
+        nx = Nbath + Nbath*Norb + Norb
+        allocate(x(nx), g(nx), om(nwl), dv(2*Norb*nwl))
+        read(10) x
+        read(10) om
+        read(10) dv
+        nwloc = nwl / nodes
+        j = mod(nwl, nodes)
+        if (node .lt. j) then
+            nwloc = nwloc + 1
+            i2 = node * nwloc
+        else
+            i2 = j + node * nwloc
+        endif
+        allocate(dev(Norb, nwloc))
+        do i = 1, nwloc
+            do j = 1, Norb
+                dev(j, i) = dcmplx(dv(2*(j-1+Norb*(i2+i-1))+1), dv(2*(j-1+Norb*(i2+i-1))+2))
+            enddo
+        enddo
+        call mpi_barrier(comm, ierr)
+        t0 = mpi_wtime()
+        call minimization(x, nwloc, Nsite, nx, om(i2+1:i2+nwloc), dev, Nbath, Norb, comm, fret)
+        call mpi_barrier(comm, ierr)
+        t1 = mpi_wtime()
+        if (node .eq. 0) then
+            write(11) t1 - t0
+            write(11) nodes
+            write(11) fret
+            write(11) x
+        endif
     end select
     if (node .eq. 0) close(11)
     close(10)
