@@ -627,13 +627,15 @@ hipError_t launch_lzb_dot(const EigArgs& e, const double* a, const double* b, co
 
 // After step m - 1 (basis size m) of the problems still running: the lowest nev Ritz pairs of T_m
 // pass |beta_m z_{m,i}| <= tol |theta_i| (ritz_converged) -> done, mfinal = m.  One wave per problem.
-__global__ void __launch_bounds__(64) eig_conv_kernel(EigArgs e, int m, int nev, double tol, double* zs) {
-    extern __shared__ double t3[];  // 3 ldh: T_m staged
+__global__ void __launch_bounds__(64) eig_conv_kernel(EigArgs e, int m, int nev, double tol, double* zs,
+                                                      int ws_lds) {
+    extern __shared__ double t3[];  // 3 ldh: T_m staged; then the workspace when it fits
     const int p = blockIdx.x;
     if (e.done[p] || m < nev) return;
     const int ldh = e.ldh;
+    double* ws = ws_lds ? t3 + 3 * ldh : zs + (size_t)p * nev * 3 * ldh;
     const bool conv = ritz_converged(e.alpha + (size_t)p * ldh, e.beta + (size_t)p * ldh, m, nev, tol, t3, t3 + ldh,
-                                     t3 + 2 * ldh, zs + (size_t)p * nev * 3 * ldh, ldh);
+                                     t3 + 2 * ldh, ws, ldh);
     if (threadIdx.x == 0 && conv) {
         e.done[p] = 1;
         e.mfinal[p] = m;
@@ -641,8 +643,17 @@ __global__ void __launch_bounds__(64) eig_conv_kernel(EigArgs e, int m, int nev,
 }
 
 hipError_t launch_eig_conv(const EigArgs& e, int m, int nev, double tol, double* zs, hipStream_t s) {
-    const size_t lds = (size_t)3 * e.ldh * sizeof(double);
-    hipLaunchKernelGGL(eig_conv_kernel, dim3(e.P), dim3(64), lds, s, e, m, nev, tol, zs);
+    // the inverse-iteration workspace (at most 64 eigenvalues at a time) in LDS where it fits
+    const size_t ws = (size_t)std::min(nev, 64) * 3 * e.ldh * sizeof(double);
+    const size_t t3 = (size_t)3 * e.ldh * sizeof(double);
+    const int in_lds = t3 + ws <= 96 * 1024;
+    const size_t lds = t3 + (in_lds ? ws : 0);
+    if (lds > 64 * 1024) {
+        hipError_t err = hipFuncSetAttribute((const void*)eig_conv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds);
+        if (err != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(eig_conv_kernel, dim3(e.P), dim3(64), lds, s, e, m, nev, tol, zs, in_lds);
     return hipGetLastError();
 }
 
