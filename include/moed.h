@@ -45,6 +45,9 @@ extern "C" {
                                    order instead of the reference's sequential order: deterministic and
                                    much faster, rounded differently (the CG path then differs in the
                                    last bits from the reference's) */
+#define MOED_F_NO_DEGENERACY_CHECK 64u  /* moed_solve with nev >= 2: skip the check for levels a
+                                   single-start Lanczos run misses (each degenerate level is otherwise
+                                   found with its multiplicity by deflated restarts) */
 #define MOED_F_FULL_REORTH 32u  /* eigensolver: full reorthogonalisation, two classical Gram-Schmidt
                                    passes against the whole Krylov basis at every step (CGS2), the
                                    contractions and the Ritz product on the FP64 matrix cores, instead

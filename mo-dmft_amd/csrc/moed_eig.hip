@@ -623,6 +623,30 @@ hipError_t launch_lzb_dot(const EigArgs& e, const double* a, const double* b, co
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ deflated operator
+
+// c_i = -sigma_p <L_i, V_k> / |V_k| from the item partials of eig_multidot_kernel (locked vectors
+// L_0..L_{nl-1}); one wave per problem
+__global__ void __launch_bounds__(64) eig_defl_coef_kernel(EigArgs e, int k, int nl, const double* sigma) {
+    const int p = blockIdx.x;
+    if (e.done[p]) return;
+    for (int i = 0; i < nl; ++i) {
+        const double sum = wave_item_sum(e.part, (size_t)i * e.ni + e.pitem0[p], e.pnitem[p]);
+        if (threadIdx.x == 0) e.c[(size_t)p * e.ldh + i] = -sigma[p] * (sum / e.nrm[(size_t)p * e.ldh + k]);
+    }
+}
+
+hipError_t launch_eig_deflate(const EigArgs& e, const double* const* L, int nl, const double* sigma,
+                              const double* vk, double* w, int k, hipStream_t s) {
+    if (nl <= 0) return hipSuccess;
+    const int ng = (nl + kEigVG - 1) / kEigVG;
+    hipLaunchKernelGGL(eig_multidot_kernel, dim3(e.ni, ng), dim3(kEigThreads), 0, s, e, L, vk, 0, nl, 0);
+    hipLaunchKernelGGL(eig_defl_coef_kernel, dim3(e.P), dim3(64), 0, s, e, k, nl, sigma);
+    hipLaunchKernelGGL(eig_project_kernel, dim3(e.ni), dim3(kEigThreads), (size_t)nl * sizeof(double), s, e, L, w, w,
+                       0, nl, 0);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ convergence on the device
 
 // After step m - 1 (basis size m) of the problems still running: the lowest nev Ritz pairs of T_m

@@ -641,8 +641,16 @@ constexpr size_t kBasisBytes = (size_t)4 << 30;
 // persistent: problems small enough for one workgroup (D <= kPersistMaxDim, at most kPersistMaxU up
 // strings) run their whole iteration in one persistent work-queue launch (moed_persist.hip) instead
 // of step by step; the rest go step by step as before.
+// Deflation (degeneracy check): the iteration runs on H' = H + sigma_p sum_i L_i L_i^T, with the
+// locked vectors L_i packed like the basis (zero where a problem has fewer than nl).
+struct Deflation {
+    const double* const* L;  // device table of nl packed vectors
+    int nl;
+    const double* sigma;     // device [P]
+};
+
 int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out,
-              bool persistent = false) {
+              bool persistent = false, const Deflation* defl = nullptr, uint64_t seed_salt = 0) {
     hipStream_t st = h->stream;
     const int P = (int)secs.size();
     if (P == 0) return MOED_OK;
@@ -813,8 +821,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     double* W = g[EG_W].as<double>();
     for (int p = 0; p < P; ++p)
         if (!pers[p])
-            HIPCHK(launch_fill_start(0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second, dim[p],
-                                     vp[0] + off[p], st));
+            HIPCHK(launch_fill_start((0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second) ^ seed_salt,
+                                     dim[p], vp[0] + off[p], st));
     HIPCHK(launch_eig_segdot(e, vp[0], vp[0], d_nrm, ldh, 1, st));  // nrm[p][0] = |u_0|
 
     HvBatchArgs hb{};
@@ -846,7 +854,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         // the whole iteration of every queued problem in one launch; its basis is allocated up front
         if (int rc = reserve(pers_M + 1)) return rc;
         std::vector<uint64_t> seeds(P);
-        for (int p = 0; p < P; ++p) seeds[p] = 0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second;
+        for (int p = 0; p < P; ++p)
+            seeds[p] = (0x5eed0000ull + (uint64_t)secs[p].first * 131 + secs[p].second) ^ seed_salt;
         const int32_t zero2[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(d_pqueue, pqueue.data(), pqueue.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(d_head, zero2, sizeof(zero2), hipMemcpyHostToDevice, st));
@@ -889,6 +898,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
             int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
             if (rc) return rc;
         }
+        if (defl) HIPCHK(launch_eig_deflate(e, defl->L, defl->nl, defl->sigma, Vk, W, k, st));
         HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 0, 0.5, 1e-14, st));  // three-term part
         HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 1, 0.5, 1e-14, st));  // whole basis
         HIPCHK(launch_eig_pass(e, vt, vp[k + 1], W, k, 2, 0.5, 1e-14, st));  // again where DGKS asks
@@ -995,6 +1005,92 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     HIPCHK(hipMemcpyAsync(g[EG_OUT].p, outp.data(), P * sizeof(double*), hipMemcpyHostToDevice, st));
     HIPCHK(launch_eig_ritz(e, vt, d_S, ldh, nev, d_nf, g[EG_OUT].as<double*>(), d_n2, st));
     HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
+// Levels a single-start Lanczos run can miss: its Krylov space holds one direction per eigenspace,
+// so a degenerate level (identical orbitals, for one) is found once.  For every sector solved by
+// Lanczos with nev >= 2: run the iteration again, from another start vector, on H' = H + sigma
+// sum_i y_i y_i^T with the sector's Ritz vectors y_i shifted out of the way (sigma > the spread of
+// the found values), for its lowest Ritz pair only.  A value below the sector's nev-th (by more than
+// 1e-9 relative) is a missed level: it joins the sector's list with its vector, and the check runs
+// again; a sector is done when its check finds nothing below.  Every sector's check runs in one
+// multi-problem run.
+int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
+    hipStream_t st = h->stream;
+    for (int round = 0; round < 2 * nev + 2 && !cand.empty(); ++round) {
+        const int P = (int)cand.size();
+        std::vector<std::pair<int, int>> secs(P);
+        std::vector<int64_t> off(P);
+        int64_t dtot = 0;
+        int nl = 1;
+        for (int p = 0; p < P; ++p) {
+            const SectorEig& E = *h->eig[cand[p]];
+            secs[p] = {E.nu, E.nd};
+            off[p] = dtot;  // the engine's packed layout: every problem on an even element
+            dtot += E.D + (E.D & 1);
+            nl = std::max(nl, E.nfound);
+        }
+        DevBuf& Lb = h->ws[WS_MISC];
+        HIPCHK(Lb.ensure((size_t)nl * dtot * sizeof(double) + (size_t)nl * sizeof(double*) + P * sizeof(double)));
+        double* L = Lb.as<double>();
+        double** Lt = reinterpret_cast<double**>(L + (size_t)nl * dtot);
+        double* sig = reinterpret_cast<double*>(Lt + nl);
+        HIPCHK(launch_fill(0.0, (int64_t)nl * dtot, L, st));
+        std::vector<double*> lt(nl);
+        std::vector<double> sg(P);
+        for (int i = 0; i < nl; ++i) lt[i] = L + (size_t)i * dtot;
+        for (int p = 0; p < P; ++p) {
+            const SectorEig& E = *h->eig[cand[p]];
+            for (int i = 0; i < E.nfound; ++i)
+                HIPCHK(hipMemcpyAsync(lt[i] + off[p], E.vec.as<double>() + (size_t)i * E.D, E.D * sizeof(double),
+                                      hipMemcpyDeviceToDevice, st));
+            const double lo = E.ev[0], top = E.ev[E.nfound - 1];
+            sg[p] = (top - lo) + std::max(1.0, std::fabs(top)) + std::max(1.0, std::fabs(lo));
+        }
+        HIPCHK(hipMemcpyAsync(Lt, lt.data(), nl * sizeof(double*), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(sig, sg.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+        std::vector<std::unique_ptr<SectorEig>> chk(P);
+        std::vector<SectorEig*> outs(P);
+        for (int p = 0; p < P; ++p) {
+            chk[p] = std::make_unique<SectorEig>();
+            outs[p] = chk[p].get();
+        }
+        const Deflation d{Lt, nl, sig};
+        if (int rc = eig_batch(h, secs, 1, outs, false, &d, 0x9e3779b97f4a7c15ull * (uint64_t)(round + 1))) return rc;
+        std::vector<int> next;
+        for (int p = 0; p < P; ++p) {
+            SectorEig& E = *h->eig[cand[p]];
+            const SectorEig& C = *chk[p];
+            if (C.nfound < 1) continue;
+            const double th = C.ev[0];
+            const double top = E.nfound == nev ? E.ev[nev - 1] : DBL_MAX;
+            if (!(th < top - 1e-9 * std::max(1.0, std::fabs(top)))) continue;
+            // insert (th, its vector) in order; the list keeps the lowest nev
+            const int nf = std::min(nev, E.nfound + 1);
+            int j = 0;
+            while (j < E.nfound && E.ev[j] <= th) ++j;
+            DevBuf nv;
+            HIPCHK(nv.ensure((size_t)nf * E.D * sizeof(double)));
+            double* dst = nv.as<double>();
+            const double* src = E.vec.as<double>();
+            const size_t vb = E.D * sizeof(double);
+            if (j > 0) HIPCHK(hipMemcpyAsync(dst, src, (size_t)j * vb, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(dst + (size_t)j * E.D, C.vec.as<double>(), vb, hipMemcpyDeviceToDevice, st));
+            if (nf - j - 1 > 0)
+                HIPCHK(hipMemcpyAsync(dst + (size_t)(j + 1) * E.D, src + (size_t)j * E.D, (size_t)(nf - j - 1) * vb,
+                                      hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            std::swap(E.vec.p, nv.p);
+            std::swap(E.vec.bytes, nv.bytes);
+            std::vector<double> ev(nev, DBL_MAX);
+            for (int i = 0, q = 0; i < nf; ++i) ev[i] = i == j ? th : E.ev[q++];
+            E.ev = ev;
+            E.nfound = nf;
+            next.push_back(cand[p]);
+        }
+        cand = next;
+    }
     return MOED_OK;
 }
 
@@ -1636,6 +1732,18 @@ int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int
             const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
             int rc = eig_sector(h, nu, nd, nev, *h->eig[is]);
             if (rc) return rc;
+        }
+    }
+    if (nev >= 2 && !(h->flags & MOED_F_NO_DEGENERACY_CHECK)) {
+        // the sectors solved by Lanczos; every check in one run with MOED_F_BATCHED, else sector by sector
+        std::vector<int> cand;
+        for (int is = 0; is < nsec; ++is)
+            if (h->eig[is]->D > 64 && h->eig[is]->nfound >= 1) cand.push_back(is);
+        if (h->flags & MOED_F_BATCHED) {
+            if (int rc = degeneracy_check(h, nev, cand)) return rc;
+        } else {
+            for (int is : cand)
+                if (int rc = degeneracy_check(h, nev, {is})) return rc;
         }
     }
     for (int is = 0; is < nsec; ++is)

@@ -155,6 +155,11 @@ hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b,
 //   pass 2: (problems flagged) pass 1 again; after pass 1 or 2 the step's alpha, beta and stop test
 hipError_t launch_eig_pass(const EigArgs& e, const double* const* V, double* vk1, const double* w, int k, int pass,
                            double eta2, double tiny, hipStream_t s);
+// deflated operator H' = H + sigma_p sum_i L_i L_i^T on the step's product: w += sigma_p sum_i
+// <L_i, V_k / |V_k|> L_i for the locked vectors L_0..L_{nl-1} (packed like w; zero vectors where a
+// problem has fewer), w = H V_k / |V_k| on entry
+hipError_t launch_eig_deflate(const EigArgs& e, const double* const* L, int nl, const double* sigma,
+                              const double* vk, double* w, int k, hipStream_t s);
 // convergence after a step with basis size m, on the device: for every running problem, the lowest
 // nev Ritz pairs of T_m pass |beta_m z_{m,i}| <= tol |theta_i| -> done = 1, mfinal = m
 // (zs: [P][nev][3 ldh] workspace)

@@ -30,6 +30,7 @@ MOED_F_EXACT_CMPLX = 1
 MOED_F_BATCHED = 8
 MOED_F_FIT_TREE = 16
 MOED_F_FULL_REORTH = 32
+MOED_F_NO_DEGENERACY_CHECK = 64
 
 MOED_EARG, MOED_ESECTOR, MOED_EZERO, MOED_EHIP, MOED_ENOMEM, MOED_ESTATE, MOED_ECONV = -1, -2, -3, -4, -5, -6, -7
 _ERR = {-1: "EARG", -2: "ESECTOR", -3: "EZERO", -4: "EHIP", -5: "ENOMEM", -6: "ESTATE", -7: "ECONV"}

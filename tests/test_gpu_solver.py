@@ -489,3 +489,72 @@ def test_lanczos_graph_replays_are_bit_identical(configs):
     b54 = oracle.Basis(c.nsite, 5, 4)
     nsco, ao, bo = oracle.lanczos(m, b54, start_vector(104, b54.ntot), 60)
     assert np.allclose(out[(5, 4)][0][1][:20], ao[:20], rtol=1e-10, atol=1e-12)
+
+
+def _degenerate_model():
+    """two orbitals with identical baths: every level of an odd-N sector is (at least) doubly
+    degenerate"""
+    norb, nbath = 2, 4
+    eb, vb = np.array([-0.9, 0.6]), np.array([0.45, 0.3])
+    ek = np.concatenate([np.full(norb, -1.2), eb, eb])
+    vk = np.zeros((norb, nbath))
+    vk[0, :2] = vb
+    vk[1, 2:] = vb
+    return norb, nbath, 3.0, 0.4, ek, vk
+
+
+@pytest.mark.parametrize("flags", [0, moed.MOED_F_BATCHED])
+def test_degenerate_levels_found_with_multiplicity(flags):
+    """identical orbitals, every sector of the model, nev = 4: the eigenvalues of every sector solved
+    by Lanczos (D > 64) equal dense ED's lowest four with their multiplicities (the deflated
+    restarts of the degeneracy check find the copies a single start vector misses)"""
+    norb, nbath, U, J, ek, vk = _degenerate_model()
+    ns = norb + nbath
+    sectors = [(a, b) for a in range(ns + 1) for b in range(ns + 1)]
+    nev = 4
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=50.0, nev=nev, nstep=40, sectors=sectors, flags=flags) as s:
+        s.ed_solve()
+        evs = [s.sector_eigenvalues(i + 1) for i in range(len(sectors))]
+    ndeg = 0
+    for (nu, nd), ev in zip(sectors, evs):
+        sec = dense_ed.Sector(ns, nu, nd)
+        w = np.linalg.eigvalsh(dense_ed.hamiltonian(sec, norb, nbath, U, J, ek, vk).toarray())
+        k = min(nev, len(w))
+        assert np.allclose(ev[:k], w[:k], rtol=0, atol=1e-9 * max(1, abs(w[0]))), (nu, nd, ev[:k], w[:k])
+        if sec.dim > 64 and abs(w[1] - w[0]) < 1e-9:
+            ndeg += 1
+    assert ndeg >= 4  # the model does have degenerate ground levels in Lanczos-sized sectors
+
+
+def test_degenerate_ground_pair_green_against_lehmann():
+    """sector (2, 1) of the identical-orbital model (D = 90): its degenerate ground pair is kept with
+    p = 1/2 each, and G(iw) of both orbitals (both spins, factor 1/2, ed_green.F90:87-94) equals the
+    Lehmann sum averaged over the pair, so the two orbitals' G agree"""
+    norb, nbath, U, J, ek, vk = _degenerate_model()
+    ns = norb + nbath
+    om = omega_grid(64, 1000.0)
+    with EDSolver(norb, nbath, U, J, ek, vk, beta=1000.0, nev=4, nstep=200, sectors=[(2, 1)],
+                  flags=MOED_F_EXACT_CMPLX) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(om)
+    s0 = dense_ed.Sector(ns, 2, 1)
+    w, V = np.linalg.eigh(dense_ed.hamiltonian(s0, norb, nbath, U, J, ek, vk).toarray())
+    assert abs(w[1] - w[0]) < 1e-10 and w[2] - w[1] > 0.05
+    assert len(kept) == 2 and all(abs(k[1] - 0.5) < 1e-9 for k in kept)
+    z = 1j * om
+    for orb in range(norb):
+        g = np.zeros(len(om), complex)
+        for st in range(2):
+            psi = V[:, st]
+            for spin in (0, 1):
+                du, dd = (1, 0) if spin == 0 else (0, 1)
+                sp_ = dense_ed.Sector(ns, 2 + du, 1 + dd)
+                ep, vp = np.linalg.eigh(dense_ed.hamiltonian(sp_, norb, nbath, U, J, ek, vk).toarray())
+                amp = vp.T @ (dense_ed.c_matrix(s0, sp_, orb, spin, True) @ psi)
+                g += 0.5 * 0.5 * np.sum((amp ** 2)[:, None] / (z[None, :] - (ep - w[0])[:, None]), axis=0)
+                sm = dense_ed.Sector(ns, 2 - du, 1 - dd)
+                em, vm = np.linalg.eigh(dense_ed.hamiltonian(sm, norb, nbath, U, J, ek, vk).toarray())
+                amp = vm.T @ (dense_ed.c_matrix(s0, sm, orb, spin, False) @ psi)
+                g += 0.5 * 0.5 * np.sum((amp ** 2)[:, None] / (z[None, :] + (em - w[0])[:, None]), axis=0)
+        assert np.max(np.abs(G[orb] - g)) <= 1e-8 * np.max(np.abs(g)), orb
+    assert np.max(np.abs(G[0] - G[1])) <= 1e-8 * np.max(np.abs(G[0]))
