@@ -649,21 +649,39 @@ hipError_t launch_eig_deflate(const EigArgs& e, const double* const* L, int nl, 
 
 // ------------------------------------------------------------------ convergence on the device
 
-// After step m - 1 (basis size m) of the problems still running: the lowest nev Ritz pairs of T_m
-// pass |beta_m z_{m,i}| <= tol |theta_i| (ritz_converged) -> done, mfinal = m.  One wave per problem.
+// After step m - 1 (basis size m): the lowest nev Ritz pairs of T_m pass |beta_m z_{m,i}| <= tol
+// |theta_i| or lie above e.above[p] (ritz_converged) -> cflag[p] = m.  One wave per problem.  The
+// test runs on its own stream beside the steps that follow: it reads alpha, beta[0, m), which no
+// later step writes, and it leaves done and mfinal to launch_eig_conv_apply on the steps' stream.
+// A problem already stopped at a basis size <= m (exhaustion or cap, mfinal > 0) is not tested; a
+// later step's stop (mfinal > m) does not keep the test from passing at m.
 __global__ void __launch_bounds__(64) eig_conv_kernel(EigArgs e, int m, int nev, double tol, double* zs,
                                                       int ws_lds) {
     extern __shared__ double t3[];  // 3 ldh: T_m staged; then the workspace when it fits
     const int p = blockIdx.x;
-    if (e.done[p] || m < nev) return;
+    if (m < nev || e.cflag[p] != 0) return;
+    const int mf = __atomic_load_n(e.mfinal + p, __ATOMIC_RELAXED);
+    if (mf > 0 && mf <= m) return;
     const int ldh = e.ldh;
     double* ws = ws_lds ? t3 + 3 * ldh : zs + (size_t)p * nev * 3 * ldh;
     const bool conv = ritz_converged(e.alpha + (size_t)p * ldh, e.beta + (size_t)p * ldh, m, nev, tol, t3, t3 + ldh,
-                                     t3 + 2 * ldh, ws, ldh);
-    if (threadIdx.x == 0 && conv) {
+                                     t3 + 2 * ldh, ws, ldh, e.above ? e.above[p] : INFINITY);
+    if (threadIdx.x == 0 && conv) e.cflag[p] = m;
+}
+
+__global__ void eig_conv_apply_kernel(EigArgs e) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= e.P) return;
+    const int c = e.cflag[p];
+    if (c > 0 && (!e.done[p] || e.mfinal[p] > c)) {
         e.done[p] = 1;
-        e.mfinal[p] = m;
+        e.mfinal[p] = c;
     }
+}
+
+hipError_t launch_eig_conv_apply(const EigArgs& e, hipStream_t s) {
+    hipLaunchKernelGGL(eig_conv_apply_kernel, dim3((e.P + 255) / 256), dim3(256), 0, s, e);
+    return hipGetLastError();
 }
 
 hipError_t launch_eig_conv(const EigArgs& e, int m, int nev, double tol, double* zs, hipStream_t s) {

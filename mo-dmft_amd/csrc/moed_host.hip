@@ -18,6 +18,7 @@
 
 #include "moed_fit.h"
 #include "moed_internal.h"
+#include "moed_ritzconv.h"
 #include "tridiag.h"
 
 using namespace moed;
@@ -102,6 +103,9 @@ struct GreenRecord {
     std::vector<double> ap, bp, an, bn;
 };
 
+// done-flag copies of the eigensolver's convergence tests in flight at once
+constexpr int kFlagRing = 4;
+
 }  // namespace
 
 struct moed_handle {
@@ -135,6 +139,10 @@ struct moed_handle {
     int32_t* pinned_i32 = nullptr;  // pinned host buffer for flags read back by the eigensolver
     hipStream_t stream2 = nullptr;  // second stream (default priority): the persistent eigensolver
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream (default priority): the eigensolver's convergence tests beside its steps
+    hipStream_t stream3 = nullptr;
+    hipEvent_t ev_step = nullptr, ev_convd = nullptr;
+    hipEvent_t ev_flags[kFlagRing] = {};  // done-flag copies to pinned memory in flight
     int64_t persist_max = -1;       // largest sector dimension for the persistent launch (-1: default)
     size_t pinned_n = 0;
     int ncu = 0;  // compute units of the device
@@ -643,10 +651,14 @@ constexpr size_t kBasisBytes = (size_t)4 << 30;
 // of step by step; the rest go step by step as before.
 // Deflation (degeneracy check): the iteration runs on H' = H + sigma_p sum_i L_i L_i^T, with the
 // locked vectors L_i packed like the basis (zero where a problem has fewer than nl).
+// A problem only asks whether H' has a level below above[p]: its Ritz pairs whose residual interval
+// lies wholly above it need not converge.
 struct Deflation {
     const double* const* L;  // device table of nl packed vectors
     int nl;
     const double* sigma;     // device [P]
+    const double* above;     // device [P]
+    const double* above_h;   // the same, host [P]
 };
 
 int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int nev, std::vector<SectorEig*>& out,
@@ -716,7 +728,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     const size_t npart = std::max<size_t>((size_t)ldh * ni, (size_t)nev * ni);
     HIPCHK(g[EG_VT].ensure((size_t)(M + 1) * sizeof(double*)));
     HIPCHK(g[EG_W].ensure((size_t)dtot * sizeof(double)));
-    const size_t n32 = 2 * (size_t)ni + 9 * (size_t)P + 2 * (size_t)nr + pqueue.size() + 2;
+    const size_t n32 = 2 * (size_t)ni + 10 * (size_t)P + 2 * (size_t)nr + pqueue.size() + 2;
     HIPCHK(g[EG_I32].ensure(n32 * sizeof(int32_t)));
     HIPCHK(g[EG_I64].ensure(((size_t)ni + 3 * P) * sizeof(int64_t)));
     const size_t nzs = (size_t)P * nev * 3 * ldh;  // convergence test workspace
@@ -728,7 +740,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     int32_t *d_item_prob = i32, *d_item_len = d_item_prob + ni, *d_pitem0 = d_item_len + ni, *d_pnitem = d_pitem0 + P,
             *d_mmax = d_pnitem + P, *d_done = d_mmax + P, *d_pass2 = d_done + P, *d_mfinal = d_pass2 + P,
             *d_nf = d_mfinal + P, *d_row_prob = d_nf + P, *d_row_id = d_row_prob + nr, *d_full = d_row_id + nr,
-            *d_force = d_full + P, *d_pqueue = d_force + P, *d_head = d_pqueue + pqueue.size();
+            *d_force = d_full + P, *d_cflag = d_force + P, *d_pqueue = d_cflag + P, *d_head = d_pqueue + pqueue.size();
     int64_t *d_item_start = g[EG_I64].as<int64_t>(), *d_poff = d_item_start + ni, *d_pdim = d_poff + P,
             *d_seed = d_pdim + P;
     double* dbl = g[EG_DBL].as<double>();
@@ -748,6 +760,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     std::vector<int32_t> done0(P);
     for (int p = 0; p < P; ++p) done0[p] = pers[p];
     HIPCHK(hipMemcpyAsync(d_done, done0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    for (int p = 0; p < P; ++p) done0[p] = -done0[p];  // cflag: -1 for the persistent launch's problems
+    HIPCHK(hipMemcpyAsync(d_cflag, done0.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_pass2, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_mfinal, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_full, zeroP.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -788,6 +802,8 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     e.full = d_full;
     e.force = d_force;
     e.full_mode = (h->flags & MOED_F_FULL_REORTH) ? 1 : 0;
+    e.above = defl ? defl->above : nullptr;
+    e.cflag = d_cflag;
 
     // basis vectors: those the handle's chunks already hold, then new chunks as steps need them
     const size_t vbytes = (size_t)dtot * sizeof(double);
@@ -841,14 +857,19 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
 
     std::vector<int32_t> done_h(P, 0), mfin_h(P, 0);
     std::vector<double> al_h((size_t)P * ldh), be_h((size_t)P * ldh);
-    if (h->pinned_n < (size_t)P) {
+    // pinned: the host's view of the done flags, then kFlagRing copies in flight
+    const size_t npin = (size_t)(kFlagRing + 1) * P;
+    if (h->pinned_n < npin) {
         if (h->pinned_i32) (void)hipHostFree(h->pinned_i32);
         h->pinned_i32 = nullptr;
         h->pinned_n = 0;
-        HIPCHK(hipHostMalloc((void**)&h->pinned_i32, (size_t)P * sizeof(int32_t), hipHostMallocDefault));
-        h->pinned_n = (size_t)P;
+        HIPCHK(hipHostMalloc((void**)&h->pinned_i32, npin * sizeof(int32_t), hipHostMallocDefault));
+        h->pinned_n = npin;
     }
-    for (int p = 0; p < P; ++p) h->pinned_i32[p] = pers[p];
+    int32_t* flags = h->pinned_i32;
+    int32_t* ring = h->pinned_i32 + P;
+    for (int p = 0; p < P; ++p) flags[p] = pers[p];
+    int ntest = 0;  // device convergence tests launched
     int active = P - (int)pqueue.size();
     if (!pqueue.empty()) {
         // the whole iteration of every queued problem in one launch; its basis is allocated up front
@@ -894,7 +915,7 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         hb.scale = d_nrm + k;
         HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
         for (int p = 0; p < P; ++p) {
-            if (dim[p] <= kSmall || h->pinned_i32[p]) continue;  // flags as of the last check
+            if (dim[p] <= kSmall || flags[p]) continue;  // flags as the host last read them
             int rc = hv_dev(h, S[p], Vk + off[p], W + off[p], nullptr, d_nrm + (size_t)p * ldh + k, d_done + p, st);
             if (rc) return rc;
         }
@@ -905,13 +926,31 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
         const int m = k + 1;
         if (m < 20 || m % 5 == 0 || m == M) {
             if (P > 1) {
-                // several problems: the convergence test of every running problem on the device (one
-                // launch); the host reads back only the done flags (pinned memory, one synchronisation)
-                HIPCHK(launch_eig_conv(e, m, nev, h->gs_tol, d_zs, st));
-                HIPCHK(hipMemcpyAsync(h->pinned_i32, d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                active = 0;
-                for (int p = 0; p < P; ++p) active += h->pinned_i32[p] == 0;
+                // several problems: the convergence test of every running problem on the device, on
+                // the third stream beside the steps that follow; its verdicts enter done / mfinal in
+                // stream order at the next test (a problem that passed at m stops with mfinal = m
+                // whatever steps ran meanwhile: they write nothing below m).  The host reads the done
+                // flags one test behind, so the steps stay queued on the GPU; steps queued after
+                // every problem has stopped do nothing.
+                if (ntest > 0) {
+                    HIPCHK(hipStreamWaitEvent(st, h->ev_convd, 0));
+                    HIPCHK(launch_eig_conv_apply(e, st));
+                }
+                const int slot = ntest % kFlagRing;
+                HIPCHK(hipMemcpyAsync(ring + (size_t)slot * P, d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipEventRecord(h->ev_flags[slot], st));
+                HIPCHK(hipEventRecord(h->ev_step, st));
+                HIPCHK(hipStreamWaitEvent(h->stream3, h->ev_step, 0));
+                HIPCHK(launch_eig_conv(e, m, nev, h->gs_tol, d_zs, h->stream3));
+                HIPCHK(hipEventRecord(h->ev_convd, h->stream3));
+                ++ntest;
+                if (ntest >= 2) {
+                    const int ps = (ntest - 2) % kFlagRing;
+                    HIPCHK(hipEventSynchronize(h->ev_flags[ps]));
+                    std::memcpy(flags, ring + (size_t)ps * P, P * sizeof(int32_t));
+                    active = 0;
+                    for (int p = 0; p < P; ++p) active += flags[p] == 0;
+                }
             } else {
                 // one problem: its tridiagonal QL on the host (microseconds) beats one wave on the device
                 HIPCHK(hipMemcpyAsync(done_h.data(), d_done, P * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -933,8 +972,13 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
                     }
                     const double bk = be_h[(size_t)p * ldh + m - 1];
                     bool ok = true;
-                    for (int i = 0; i < std::min(nev, m) && ok; ++i)
-                        if (std::fabs(bk * Z[i]) > h->gs_tol * std::max(std::fabs(d[i]), 1e-300)) ok = false;
+                    const double above = defl ? defl->above_h[p] : INFINITY;
+                    for (int i = 0; i < std::min(nev, m) && ok; ++i) {
+                        const double res = std::fabs(bk * Z[i]);
+                        const bool pass = res <= h->gs_tol * std::max(std::fabs(d[i]), 1e-300) ||
+                                          (res <= kAboveTol * std::max(std::fabs(d[i]), 1.0) && d[i] - res > above);
+                        if (!pass) ok = false;
+                    }
                     conv[p] = ok;
                 });
                 bool changed = false;
@@ -954,9 +998,13 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
                     HIPCHK(hipMemcpyAsync(d_done, done_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
                     HIPCHK(hipMemcpyAsync(d_mfinal, mfin_h.data(), P * sizeof(int32_t), hipMemcpyHostToDevice, st));
                 }
-                h->pinned_i32[0] = done_h[0];
+                flags[0] = done_h[0];
             }
         }
+    }
+    if (ntest > 0) {  // the last test's verdicts
+        HIPCHK(hipStreamWaitEvent(st, h->ev_convd, 0));
+        HIPCHK(launch_eig_conv_apply(e, st));
     }
     if (!pqueue.empty()) HIPCHK(hipStreamWaitEvent(st, h->ev_join, 0));
     // final Ritz pairs: T_m of every problem on the host (in parallel), S = Z / |u_k|
@@ -970,22 +1018,24 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
     std::vector<double> S_h((size_t)P * nev * ldh, 0.0);
     std::vector<int32_t> nf(P, 0);
     std::vector<int> ok(P, 1);
+    // eigenvalues of T_m by QL, the wanted eigenvectors by inverse iteration
     auto ritz = [&](int p) {
         const int m = mfin_h[p];
-        std::vector<double> d(al_h.begin() + (size_t)p * ldh, al_h.begin() + (size_t)p * ldh + m);
-        std::vector<double> ee(be_h.begin() + (size_t)p * ldh, be_h.begin() + (size_t)p * ldh + (m - 1));
-        std::vector<double> Z((size_t)m * m, 0.0);
-        for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
-        if (!tridiag_ql(d, ee, m, &Z)) {
+        const double* al = al_h.data() + (size_t)p * ldh;
+        const double* be = be_h.data() + (size_t)p * ldh;
+        std::vector<double> d(al, al + m);
+        if (!tridiag_ql(d, std::vector<double>(be, be + (m - 1)), 0, nullptr)) {
             ok[p] = 0;
             return;
         }
         nf[p] = std::min(nev, m);
+        std::vector<double> Z;
+        tridiag_vectors(al, be, m, d.data(), nf[p], Z);
         out[p]->ev.assign(nev, DBL_MAX);
         for (int i = 0; i < nf[p]; ++i) {
             out[p]->ev[i] = d[i];
             for (int kk = 0; kk < m; ++kk)
-                S_h[((size_t)p * nev + i) * ldh + kk] = Z[(size_t)kk * m + i] / nrm_h[(size_t)p * ldh + kk];
+                S_h[((size_t)p * nev + i) * ldh + kk] = Z[(size_t)i * m + kk] / nrm_h[(size_t)p * ldh + kk];
         }
     };
     parallel_for(P, ritz);
@@ -1015,7 +1065,15 @@ int eig_batch(moed_handle* h, const std::vector<std::pair<int, int>>& secs, int 
 // the found values), for its lowest Ritz pair only.  A value below the sector's nev-th (by more than
 // 1e-9 relative) is a missed level: it joins the sector's list with its vector, and the check runs
 // again; a sector is done when its check finds nothing below.  Every sector's check runs in one
-// multi-problem run.
+// multi-problem run, which stops a sector's iteration as soon as its lowest Ritz value lies above
+// the sector's nev-th level with its whole residual interval.
+// a level found by the check joins the sector's list when it lies below this
+double insert_below(const SectorEig& E, int nev) {
+    if (E.nfound < nev) return DBL_MAX;
+    const double top = E.ev[nev - 1];
+    return top - 1e-9 * std::max(1.0, std::fabs(top));
+}
+
 int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
     hipStream_t st = h->stream;
     for (int round = 0; round < 2 * nev + 2 && !cand.empty(); ++round) {
@@ -1032,13 +1090,14 @@ int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
             nl = std::max(nl, E.nfound);
         }
         DevBuf& Lb = h->ws[WS_MISC];
-        HIPCHK(Lb.ensure((size_t)nl * dtot * sizeof(double) + (size_t)nl * sizeof(double*) + P * sizeof(double)));
+        HIPCHK(Lb.ensure((size_t)nl * dtot * sizeof(double) + (size_t)nl * sizeof(double*) + 2 * P * sizeof(double)));
         double* L = Lb.as<double>();
         double** Lt = reinterpret_cast<double**>(L + (size_t)nl * dtot);
         double* sig = reinterpret_cast<double*>(Lt + nl);
+        double* abv = sig + P;
         HIPCHK(launch_fill(0.0, (int64_t)nl * dtot, L, st));
         std::vector<double*> lt(nl);
-        std::vector<double> sg(P);
+        std::vector<double> sg(P), ab(P);
         for (int i = 0; i < nl; ++i) lt[i] = L + (size_t)i * dtot;
         for (int p = 0; p < P; ++p) {
             const SectorEig& E = *h->eig[cand[p]];
@@ -1047,16 +1106,18 @@ int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
                                       hipMemcpyDeviceToDevice, st));
             const double lo = E.ev[0], top = E.ev[E.nfound - 1];
             sg[p] = (top - lo) + std::max(1.0, std::fabs(top)) + std::max(1.0, std::fabs(lo));
+            ab[p] = insert_below(E, nev);
         }
         HIPCHK(hipMemcpyAsync(Lt, lt.data(), nl * sizeof(double*), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(sig, sg.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(abv, ab.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
         std::vector<std::unique_ptr<SectorEig>> chk(P);
         std::vector<SectorEig*> outs(P);
         for (int p = 0; p < P; ++p) {
             chk[p] = std::make_unique<SectorEig>();
             outs[p] = chk[p].get();
         }
-        const Deflation d{Lt, nl, sig};
+        const Deflation d{Lt, nl, sig, abv, ab.data()};
         if (int rc = eig_batch(h, secs, 1, outs, false, &d, 0x9e3779b97f4a7c15ull * (uint64_t)(round + 1))) return rc;
         std::vector<int> next;
         for (int p = 0; p < P; ++p) {
@@ -1064,8 +1125,7 @@ int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
             const SectorEig& C = *chk[p];
             if (C.nfound < 1) continue;
             const double th = C.ev[0];
-            const double top = E.nfound == nev ? E.ev[nev - 1] : DBL_MAX;
-            if (!(th < top - 1e-9 * std::max(1.0, std::fabs(top)))) continue;
+            if (!(th < ab[p])) continue;
             // insert (th, its vector) in order; the list keeps the lowest nev
             const int nf = std::min(nev, E.nfound + 1);
             int j = 0;
@@ -1520,6 +1580,15 @@ int fit_minimize_host(moed_handle* h, int nfit, int nx, double* x, int nw, const
 
 extern "C" {
 
+static void destroy_streams(moed_handle* h) {
+    for (hipEvent_t& ev : h->ev_flags)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {h->ev_convd, h->ev_step, h->ev_join, h->ev_fork})
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t s : {h->stream3, h->stream2, h->stream})
+        if (s) (void)hipStreamDestroy(s);
+}
+
 int moed_create(const moed_params* p, moed_handle** out) {
     if (!p || !out) return MOED_EARG;
     *out = nullptr;
@@ -1554,13 +1623,15 @@ int moed_create(const moed_params* p, moed_handle** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream3, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_convd, hipEventDisableTiming);
+    for (int i = 0; i < kFlagRing && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&h->ev_flags[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e == hipSuccess && fit_scratch_address(h->device, &h->fit_scratch) != MOED_OK) e = hipErrorInvalidValue;
     if (e != hipSuccess) {  // (device is current: fit_scratch_address looks up this GPU's array)
-        if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-        if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-        if (h->stream2) (void)hipStreamDestroy(h->stream2);
-        if (h->stream) (void)hipStreamDestroy(h->stream);
+        destroy_streams(h);
         delete h;
         return e == hipErrorOutOfMemory ? MOED_ENOMEM : MOED_EHIP;
     }
@@ -1586,10 +1657,8 @@ int moed_destroy(moed_handle* h) {
     h->d_ek.release();
     h->d_coef.release();
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
-    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream3) (void)hipStreamSynchronize(h->stream3);
+    destroy_streams(h);
     delete h;
     return MOED_OK;
 }

@@ -142,6 +142,10 @@ struct EigArgs {
     int32_t* force;            // [P] 1: the next step does too (the step after a triggered one)
     int32_t full_mode;         // 1: every step projects twice against the whole basis (CGS2) on the
                                // matrix cores, and the Ritz product runs there too
+    const double* above;       // [P] or null: a Ritz pair whose residual interval lies above above[p]
+                               // needs no convergence (the caller asks only what lies below)
+    int32_t* cflag;            // [P] basis size at which the device convergence test passed (0: not
+                               // yet; -1: a problem of the persistent launch, tested there)
 };
 // part[item] = sum over the item of a*b ; then out[p * stride] = sum over p's items (ordered) (op 1: sqrt)
 hipError_t launch_eig_segdot(const EigArgs& e, const double* a, const double* b, double* out, int stride, int op,
@@ -164,6 +168,9 @@ hipError_t launch_eig_deflate(const EigArgs& e, const double* const* L, int nl, 
 // nev Ritz pairs of T_m pass |beta_m z_{m,i}| <= tol |theta_i| -> done = 1, mfinal = m
 // (zs: [P][nev][3 ldh] workspace)
 hipError_t launch_eig_conv(const EigArgs& e, int m, int nev, double tol, double* zs, hipStream_t s);
+// the verdicts of launch_eig_conv in stream order: done = 1, mfinal = cflag where cflag > 0 and the
+// problem has not stopped at or before that basis size
+hipError_t launch_eig_conv_apply(const EigArgs& e, hipStream_t s);
 // Ritz vectors: out[p] + j*dim_p = sum_{i < mfinal[p]} S[(p*nev + j)*lds + i] V_i, j < nf[p], normalised
 hipError_t launch_eig_ritz(const EigArgs& e, const double* const* V, const double* S, int lds, int nev,
                            const int32_t* nf, double* const* out, double* n2, hipStream_t s);

@@ -26,12 +26,18 @@ __device__ __forceinline__ int sturm_count(const double* a, const double* b2, in
 // Called by all 64 lanes of one wave.  alpha[0..m), beta[0..m) in global memory (beta[m-1] is the
 // residual norm of the step); ta, tb, tb2 hold m doubles each (LDS or global) for the staged T_m;
 // ws holds nev * 3 * ldh doubles of inverse-iteration workspace (LDS where it fits).  Returns the
-// same verdict in every lane: true when every one of the lowest min(nev, m) Ritz pairs passes.
+// same verdict in every lane: true when every one of the lowest min(nev, m) Ritz pairs passes, or
+// lies above `above` with its whole residual interval at a residual below kAboveTol |theta_i| (a
+// caller asking only which levels lie below `above` needs no more of it; the residual bound keeps
+// an early, unresolved Ritz value from passing: a level that the Krylov space has not yet resolved
+// still weighs about 1/sqrt(dim) in the Ritz vector, and its gap times that in the residual).
 // Eigenvalues by multisection: the lanes form one group of P (a power of two) per wanted
 // eigenvalue, and each round evaluates P Sturm counts inside the group's bracket and keeps the
 // sub-interval that holds the eigenvalue, a factor P + 1 per round, down to adjacent doubles.
+constexpr double kAboveTol = 1e-8;
+
 __device__ bool ritz_converged(const double* alpha, const double* beta, int m, int nev, double tol, double* ta,
-                               double* tb, double* tb2, double* ws, int ldh) {
+                               double* tb, double* tb2, double* ws, int ldh, double above = INFINITY) {
     const int lane = threadIdx.x & 63;
     const double eps = 0x1p-52;
     for (int j = lane; j < m; j += 64) {
@@ -68,9 +74,9 @@ __device__ bool ritz_converged(const double* alpha, const double* beta, int m, i
         for (double f = 1.0; f < 0x1p60; f *= (double)(P + 1)) ++rounds;
         for (int it = 0; it < rounds; ++it) {
             const double x = l + (u - l) * ((double)(jj + 1) / (double)(P + 1));
-            const bool above = act && x > l && x < u && sturm_count(ta, tb2, m, x, tiny) > ei;
-            double nu = (act && above) ? x : u;
-            double nl = (act && !above && x > l && x < u) ? x : l;
+            const bool right = act && x > l && x < u && sturm_count(ta, tb2, m, x, tiny) > ei;
+            double nu = (act && right) ? x : u;
+            double nl = (act && !right && x > l && x < u) ? x : l;
             for (int off = 1; off < P; off <<= 1) {  // within the group of P lanes
                 nu = fmin(nu, __shfl_xor(nu, off, 64));
                 nl = fmax(nl, __shfl_xor(nl, off, 64));
@@ -114,7 +120,9 @@ __device__ bool ritz_converged(const double* alpha, const double* beta, int m, i
                 for (int j = 0; j < m; ++j) r[j] *= sc;
                 zb = r[m - 1];
             }
-            ok = ok && fabs(beta[m - 1] * zb) <= tol * fmax(fabs(th), 1e-300);
+            const double res = fabs(beta[m - 1] * zb);
+            ok = ok && (res <= tol * fmax(fabs(th), 1e-300) ||
+                        (res <= kAboveTol * fmax(fabs(th), 1.0) && th - res > above));
         }
     }
     return __all(ok);

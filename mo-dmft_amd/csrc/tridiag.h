@@ -86,6 +86,104 @@ inline bool tridiag_ql(std::vector<double>& d, std::vector<double> e, int zrows,
     return true;
 }
 
+// Eigenvectors of the tridiagonal T (diagonal a[0..m), off-diagonal b[0..m-1)) for its lowest nvec
+// eigenvalues w[0..nvec) (ascending, as tridiag_ql returns them), by inverse iteration: T - w_i I
+// factored by Gaussian elimination with partial pivoting (Golub & Van Loan, section 4.3.3, band
+// form: U gains a second superdiagonal where rows are exchanged; a pivot below eps |T| is replaced
+// by +-eps |T|), three solves from a fixed start vector, each followed by modified Gram-Schmidt
+// against the vectors already found for eigenvalues closer than 1e-3 |T| (the cluster tolerance of
+// LAPACK's dstein); shifts closer than 10 eps |T| to the previous one are moved up by that much.
+// V[i * m + j] = component j of vector i, unit norm, sign such that its largest component is
+// positive.  O(nvec m) work against O(m^3) for accumulating every rotation of the QL iteration.
+inline void tridiag_vectors(const double* a, const double* b, int m, const double* w, int nvec,
+                            std::vector<double>& V) {
+    V.assign((size_t)nvec * m, 0.0);
+    double tn = 0.0;
+    for (int j = 0; j < m; ++j)
+        tn = std::max(tn, std::fabs(a[j]) + (j > 0 ? std::fabs(b[j - 1]) : 0.0) + (j < m - 1 ? std::fabs(b[j]) : 0.0));
+    const double eps = 2.220446049250313e-16;
+    const double tiny = std::max(eps * tn, 1e-300), ortol = 1e-3 * tn;
+    std::vector<double> dl(m), dg(m), du(m), du2(m), x(m);
+    std::vector<int> piv(m);
+    double prev = 0.0;
+    int c0 = 0;  // first vector of the current cluster
+    for (int i = 0; i < nvec; ++i) {
+        double th = w[i];
+        if (i > 0) {
+            if (th - w[i - 1] > ortol) c0 = i;
+            if (th - prev < 10.0 * tiny) th = prev + 10.0 * tiny;
+        }
+        prev = th;
+        for (int j = 0; j < m; ++j) {
+            dg[j] = a[j] - th;
+            if (j < m - 1) dl[j] = du[j] = b[j];
+            du2[j] = 0.0;
+        }
+        for (int j = 0; j < m - 1; ++j) {
+            if (std::fabs(dg[j]) >= std::fabs(dl[j])) {
+                piv[j] = 0;
+                const double f = dg[j] != 0.0 ? dl[j] / dg[j] : 0.0;
+                dl[j] = f;
+                dg[j + 1] -= f * du[j];
+            } else {  // exchange rows j and j + 1
+                piv[j] = 1;
+                const double f = dg[j] / dl[j];
+                dg[j] = dl[j];
+                dl[j] = f;
+                const double t = du[j];
+                du[j] = dg[j + 1];
+                dg[j + 1] = t - f * dg[j + 1];
+                if (j < m - 2) {
+                    du2[j] = du[j + 1];
+                    du[j + 1] = -f * du[j + 1];
+                }
+            }
+        }
+        for (int j = 0; j < m; ++j)
+            if (std::fabs(dg[j]) < tiny) dg[j] = dg[j] < 0.0 ? -tiny : tiny;
+        for (int j = 0; j < m; ++j) x[j] = 1.0 + 0.25 * std::sin(0.7 * j + 0.3 * i);  // fixed start
+        for (int it = 0; it < 3; ++it) {
+            for (int j = 0; j < m - 1; ++j) {  // P, L
+                if (piv[j] == 0) {
+                    x[j + 1] -= dl[j] * x[j];
+                } else {
+                    const double t = x[j];
+                    x[j] = x[j + 1];
+                    x[j + 1] = t - dl[j] * x[j];
+                }
+            }
+            for (int j = m - 1; j >= 0; --j) {  // U
+                double r = x[j];
+                if (j < m - 1) r -= du[j] * x[j + 1];
+                if (j < m - 2) r -= du2[j] * x[j + 2];
+                x[j] = r / dg[j];
+            }
+            for (int q = c0; q < i; ++q) {  // against the cluster's vectors found so far
+                const double* v = V.data() + (size_t)q * m;
+                double dt = 0.0;
+                for (int j = 0; j < m; ++j) dt += v[j] * x[j];
+                for (int j = 0; j < m; ++j) x[j] -= dt * v[j];
+            }
+            double mx = 0.0;
+            for (int j = 0; j < m; ++j) mx = std::max(mx, std::fabs(x[j]));
+            if (!(mx > 0.0) || !std::isfinite(mx)) {  // (lost in the cluster: restart from a unit vector)
+                for (int j = 0; j < m; ++j) x[j] = j == (i + it) % m ? 1.0 : 0.0;
+                continue;
+            }
+            for (int j = 0; j < m; ++j) x[j] /= mx;  // (keeps the next solve in range)
+            double n2 = 0.0;
+            for (int j = 0; j < m; ++j) n2 += x[j] * x[j];
+            const double sc = 1.0 / std::sqrt(n2);
+            for (int j = 0; j < m; ++j) x[j] *= sc;
+        }
+        int jm = 0;
+        for (int j = 1; j < m; ++j)
+            if (std::fabs(x[j]) > std::fabs(x[jm])) jm = j;
+        const double sg = x[jm] < 0.0 ? -1.0 : 1.0;
+        for (int j = 0; j < m; ++j) V[(size_t)i * m + j] = sg * x[j];
+    }
+}
+
 // A: n x n symmetric (row-major), overwritten.  w ascending, V column j = eigenvector j (row-major n x n).
 inline void jacobi_eig(std::vector<double> A, int n, std::vector<double>& w, std::vector<double>& V) {
     V.assign((size_t)n * n, 0.0);
