@@ -1154,6 +1154,100 @@ int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
     return MOED_OK;
 }
 
+// Every sector of at most 64 states of a solve at once: their matrices from one batched product
+// (each column of each such sector a problem of hv_batch_kernel applied to a unit vector; its terms
+// per element are those of the two-pass product, so H is the one eig_dense builds), then the dense
+// eigensolver of eig_dense on the host, the sectors in parallel.
+int eig_dense_batch(moed_handle* h, const std::vector<int>& secs, int nev) {
+    if (secs.empty()) return MOED_OK;
+    hipStream_t st = h->stream;
+    const int ns = (int)secs.size();
+    std::vector<Sector> S(ns);
+    std::vector<BatchProbDev> desc;
+    std::vector<int32_t> row_prob, row_id;
+    std::vector<int64_t> off0(ns);
+    int64_t tot = 0;
+    int max_nU = 1;
+    for (int q = 0; q < ns; ++q) {
+        const int is = secs[q];
+        if (int rc = get_sector(h, h->nup[is], h->nelec[is] - h->nup[is], &S[q])) return rc;
+        const int64_t D = S[q].D, ld = D + (D & 1);  // every column on an even element
+        off0[q] = tot;
+        for (int64_t j = 0; j < D; ++j) {
+            const int pid = (int)desc.size();
+            desc.push_back({S[q].up->dev(), S[q].dn->dev(), tot + j * ld});
+            for (int id = 0; id < S[q].dn->count; ++id) {
+                row_prob.push_back(pid);
+                row_id.push_back(id);
+            }
+        }
+        tot += D * ld;
+        max_nU = std::max(max_nU, S[q].up->count);
+    }
+    std::vector<double> X((size_t)tot, 0.0);
+    for (int q = 0; q < ns; ++q) {
+        const int64_t D = S[q].D, ld = D + (D & 1);
+        for (int64_t j = 0; j < D; ++j) X[(size_t)(off0[q] + j * ld + j)] = 1.0;
+    }
+    const int nr = (int)row_prob.size();
+    DevBuf& Xd = h->ws[WS_X];
+    DevBuf& Yd = h->ws[WS_Y];
+    DevBuf& Md = h->ws[WS_MISC];
+    HIPCHK(Xd.ensure((size_t)tot * sizeof(double)));
+    HIPCHK(Yd.ensure((size_t)tot * sizeof(double)));
+    const size_t dbytes = desc.size() * sizeof(BatchProbDev);
+    HIPCHK(Md.ensure(dbytes + 2 * (size_t)nr * sizeof(int32_t)));
+    int32_t* d_rp = reinterpret_cast<int32_t*>(static_cast<char*>(Md.p) + dbytes);
+    HIPCHK(hipMemcpyAsync(Xd.p, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(Md.p, desc.data(), dbytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_rp, row_prob.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_rp + nr, row_id.data(), nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HvBatchArgs hb{};
+    hb.prob = Md.as<BatchProbDev>();
+    hb.row_prob = d_rp;
+    hb.row_id = d_rp + nr;
+    hb.x = Xd.as<double>();
+    hb.y = Yd.as<double>();
+    hb.ek = h->d_ek.as<double>();
+    hb.slotc = h->d_slotc.as<double>();
+    hb.U = h->U;
+    hb.UmJ3 = h->UmJ3;
+    hb.Um2J = h->Um2J;
+    hb.J = h->J;
+    HIPCHK(launch_hv_batch(hb, nr, max_nU, h->norb, (int)h->slots.size(), st));
+    std::vector<double> Y((size_t)tot);
+    HIPCHK(hipMemcpyAsync(Y.data(), Yd.p, Y.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<std::vector<double>> vecs(ns);
+    parallel_for(ns, [&](int q) {
+        SectorEig& E = *h->eig[secs[q]];
+        const int D = (int)S[q].D, ld = D + (D & 1);
+        std::vector<double> Hm((size_t)D * D), w, Vm;  // column j of H; H is symmetric: row-major
+        for (int j = 0; j < D; ++j)
+            std::copy(Y.begin() + off0[q] + (size_t)j * ld, Y.begin() + off0[q] + (size_t)j * ld + D,
+                      Hm.begin() + (size_t)j * D);
+        jacobi_eig(Hm, D, w, Vm);
+        const int nf = std::min(nev, D);
+        E.nu = S[q].nu;
+        E.nd = S[q].nd;
+        E.D = D;
+        E.nfound = nf;
+        E.ev.assign(nev, DBL_MAX);
+        vecs[q].resize((size_t)nf * D);
+        for (int i = 0; i < nf; ++i) {
+            E.ev[i] = w[i];
+            for (int r = 0; r < D; ++r) vecs[q][(size_t)i * D + r] = Vm[(size_t)r * D + i];
+        }
+    });
+    for (int q = 0; q < ns; ++q) {
+        SectorEig& E = *h->eig[secs[q]];
+        HIPCHK(E.vec.ensure(vecs[q].size() * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(E.vec.p, vecs[q].data(), vecs[q].size() * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return MOED_OK;
+}
+
 int eig_sector(moed_handle* h, int nu, int nd, int nev, SectorEig& E) {
     Sector s;
     int rc = get_sector(h, nu, nd, &s);
@@ -1773,35 +1867,41 @@ int moed_solve(moed_handle* h, int32_t* nev_calc, double* val, double* prob, int
     (void)hipSetDevice(h->device);
     const int nsec = (int)h->nelec.size(), nev = h->nev;
     if (nsec < 1 || nev < 1) return fail(h, MOED_EARG, "solve: need nsector >= 1 and nev >= 1");
-    h->eig.clear();
     h->solved = false;
     std::vector<double> all((size_t)nsec * nev);
-    for (int is = 0; is < nsec; ++is) h->eig.push_back(std::make_unique<SectorEig>());
+    // the sectors' results of the last solve are overwritten in place: their vector buffers are
+    // kept (a device free synchronises the device, and most solves of a run have the same shapes)
+    if ((int)h->eig.size() != nsec) {
+        h->eig.clear();
+        for (int is = 0; is < nsec; ++is) h->eig.push_back(std::make_unique<SectorEig>());
+    }
+    for (auto& E : h->eig) {
+        E->nfound = 0;
+        E->ev.assign(nev, DBL_MAX);
+    }
+    std::vector<int> dense;
+    std::vector<std::pair<int, int>> big;
+    std::vector<int> big_is;
+    for (int is = 0; is < nsec; ++is) {
+        const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
+        Sector s;
+        if (int rc = get_sector(h, nu, nd, &s)) return rc;
+        if (s.D <= 64) {
+            dense.push_back(is);
+        } else {
+            big.push_back({nu, nd});
+            big_is.push_back(is);
+        }
+    }
+    if (int rc = eig_dense_batch(h, dense, nev)) return rc;
     if (h->flags & MOED_F_BATCHED) {
         // every sector above the dense size in one multi-problem Lanczos run
-        std::vector<std::pair<int, int>> big;
         std::vector<SectorEig*> bout;
-        for (int is = 0; is < nsec; ++is) {
-            const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
-            Sector s;
-            int rc = get_sector(h, nu, nd, &s);
-            if (rc) return rc;
-            if (s.D <= 64) {
-                rc = eig_sector(h, nu, nd, nev, *h->eig[is]);
-                if (rc) return rc;
-            } else {
-                big.push_back({nu, nd});
-                bout.push_back(h->eig[is].get());
-            }
-        }
-        int rc = eig_batch(h, big, nev, bout, true);
-        if (rc) return rc;
+        for (int is : big_is) bout.push_back(h->eig[is].get());
+        if (int rc = eig_batch(h, big, nev, bout, true)) return rc;
     } else {
-        for (int is = 0; is < nsec; ++is) {
-            const int nu = h->nup[is], nd = h->nelec[is] - h->nup[is];
-            int rc = eig_sector(h, nu, nd, nev, *h->eig[is]);
-            if (rc) return rc;
-        }
+        for (size_t q = 0; q < big.size(); ++q)
+            if (int rc = eig_sector(h, big[q].first, big[q].second, nev, *h->eig[big_is[q]])) return rc;
     }
     if (nev >= 2 && !(h->flags & MOED_F_NO_DEGENERACY_CHECK)) {
         // the sectors solved by Lanczos; every check in one run with MOED_F_BATCHED, else sector by sector

@@ -89,3 +89,85 @@ def test_tuning_table_names_workloads_and_valid_shapes():
         assert t["hv_us"][f"{t['hv_threads']}x{t['hv_strip']}"] == min(t["hv_us"].values())
         if "persistent_max_dim" in t:
             assert t["persistent_max_dim"] in autotune.PERSIST
+
+
+# ------------------------------------------------------------------ tridiagonal eigenvectors (host)
+
+_TRI_SHIM = r'''
+#include "tridiag.h"
+extern "C" int tv(const double* a, const double* b, int m, int nvec, double* w, double* V) {
+    std::vector<double> d(a, a + m);
+    if (!moed::tridiag_ql(d, std::vector<double>(b, b + (m > 0 ? m - 1 : 0)), 0, nullptr)) return 1;
+    std::vector<double> Z;
+    moed::tridiag_vectors(a, b, m, d.data(), nvec, Z);
+    for (int i = 0; i < m; ++i) w[i] = d[i];
+    for (size_t i = 0; i < Z.size(); ++i) V[i] = Z[i];
+    return 0;
+}
+'''
+
+
+def test_tridiagonal_ritz_vectors_by_inverse_iteration(tmp_path):
+    """The eigensolver's Ritz vectors of T_m (mo-dmft_amd/csrc/tridiag.h: QL eigenvalues, inverse
+    iteration for the lowest nvec vectors) against numpy's eigvalsh: residual and orthogonality to
+    rounding, also for clusters of nearly equal eigenvalues (tiny couplings, Wilkinson's matrix)
+    and for the Lanczos tridiagonal of a matrix with triply degenerate levels."""
+    import ctypes
+    import os
+    import subprocess
+    src = tmp_path / "tv.cpp"
+    src.write_text(_TRI_SHIM)
+    inc = os.path.join(os.path.dirname(__file__), "..", "mo-dmft_amd", "csrc")
+    so = tmp_path / "libtv.so"
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-I", inc, "-o", str(so), str(src)], check=True)
+    L = ctypes.CDLL(str(so))
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(7)
+
+    def lanczos_T(A, m):
+        n = A.shape[0]
+        q = rng.normal(size=n)
+        q /= np.linalg.norm(q)
+        Q, al, bl, qp, bb = [q], [], [], np.zeros(n), 0.0
+        for _ in range(min(m, n)):
+            w = A @ q - bb * qp
+            al.append(q @ w)
+            w -= al[-1] * q
+            for qq in Q:
+                w -= (qq @ w) * qq
+            bb = np.linalg.norm(w)
+            if bb < 1e-10:
+                break
+            bl.append(bb)
+            qp, q = q, w / bb
+            Q.append(q)
+        return np.array(al), np.array(bl[:len(al) - 1])
+
+    for trial in range(120):
+        m = int(rng.integers(1, 300))
+        kind = trial % 4
+        if kind == 0:
+            a, b = rng.normal(size=m), rng.uniform(0.1, 1.0, size=m - 1)
+        elif kind == 1:
+            a, b = np.abs(np.arange(m) - (m - 1) / 2.0), np.ones(m - 1)
+        elif kind == 2:
+            a, b = np.round(rng.normal(size=m), 1), rng.uniform(0.0, 1e-9, size=m - 1)
+        else:
+            n = 60
+            ev = np.concatenate([np.repeat(rng.normal(size=10), 3), rng.normal(size=30)])
+            Qm, _ = np.linalg.qr(rng.normal(size=(n, n)))
+            a, b = lanczos_T(Qm @ np.diag(ev) @ Qm.T, m)
+            m = len(a)
+        nv = min(8, m)
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b if m > 1 else np.zeros(1))
+        w, V = np.zeros(m), np.zeros(nv * m)
+        assert L.tv(a.ctypes.data_as(P), b.ctypes.data_as(P), m, nv, w.ctypes.data_as(P), V.ctypes.data_as(P)) == 0
+        V = V.reshape(nv, m)
+        T = np.diag(a) + np.diag(b[:m - 1], 1) + np.diag(b[:m - 1], -1)
+        wr = np.linalg.eigvalsh(T)
+        tn = max(np.abs(wr).max(), 1e-300)
+        assert np.abs(w - wr).max() <= 1e-13 * tn, (trial, kind)
+        for i in range(nv):
+            assert np.linalg.norm(T @ V[i] - w[i] * V[i]) <= 1e-12 * tn, (trial, kind, i)
+            assert V[i][np.argmax(np.abs(V[i]))] > 0
+        assert np.abs(V @ V.T - np.eye(nv)).max() <= 1e-10, (trial, kind)
