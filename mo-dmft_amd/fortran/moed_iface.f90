@@ -15,6 +15,7 @@ module moed_capi
     implicit none
 
     integer(c_int32_t), parameter :: MOED_F_EXACT_CMPLX = 1, MOED_F_BATCHED = 8, MOED_F_FIT_TREE = 16
+    integer(c_int32_t), parameter :: MOED_F_FULL_REORTH = 32, MOED_F_NO_DEGENERACY_CHECK = 64
 
     type, bind(C) :: moed_params
         integer(c_int32_t) :: norb, nsite
@@ -139,6 +140,16 @@ module moed_capi
             import :: c_ptr, c_int, c_int32_t
             type(c_ptr), value :: h
             integer(c_int32_t), value :: vectors
+        end function
+        integer(c_int) function moed_set_hv_shape(h, threads, strip) bind(C, name="moed_set_hv_shape")
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: threads, strip
+        end function
+        integer(c_int) function moed_set_persistent(h, max_dim) bind(C, name="moed_set_persistent")
+            import :: c_ptr, c_int, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), value :: max_dim
         end function
         integer(c_int) function moed_fit_nfev(h, nfit, nfev) bind(C, name="moed_fit_nfev")
             import :: c_ptr, c_int, c_int32_t
