@@ -178,6 +178,38 @@ def gpu_extra_timings():
     return out
 
 
+def config_solves():
+    """wall times of the other configurations' own work, timed here so that the line carries them:
+    the 14-site sector's solve and Green's functions (partial reorthogonalisation, and the solve
+    with full reorthogonalisation on the FP64 matrix cores), and one iteration of the DMFT loop at
+    the 15-site fit workload (solve, G, lattice step, 256 tree-order fits in one launch)"""
+    import moed
+    from moed import dmft
+    from moed.model import config, omega_grid
+    out = {}
+    c3 = config("ns14_roofline")
+    om = omega_grid(c3.nw, c3.beta)
+    for name, flags in (("ns14_partial_reorth", 0), ("ns14_full_reorth_mfma", moed.MOED_F_FULL_REORTH)):
+        with moed.EDSolver.from_config(c3, flags=flags) as s:
+            t0 = time.perf_counter()
+            kept = s.ed_solve()
+            t1 = time.perf_counter()
+            s.ed_calc_green_ftn(om)
+            t2 = time.perf_counter()
+        out[name] = {"t_solve_s": round(t1 - t0, 4), "t_green_s": round(t2 - t1, 4), "E0": kept[0][0]}
+    c5 = config("ns15_fit")
+    om5 = omega_grid(c5.nw, c5.beta)
+    hk = moed.band_tb(c5.norb, 10000)
+    with moed.EDSolver.from_config(c5, flags=moed.MOED_F_FIT_TREE) as s:
+        recs, _ = dmft.dmft_loop(s, hk, c5.mu, om5, 1, nfit=256, scf_tol=0.0)
+    r = recs[0]
+    out["ns15_one_dmft_iteration"] = {k: round(float(r[k]), 4) for k in ("t_solve", "t_green", "t_delta", "t_fit",
+                                                                          "t_total")}
+    out["ns15_one_dmft_iteration"].update({"E0": float(r["E0"]), "chi2": float(r["fret"]), "nfit": 256,
+                                           "fit_order": "tree"})
+    return out
+
+
 def config_of(name):
     from moed.model import config
     return config(name)
@@ -332,6 +364,7 @@ def main():
     ap.add_argument("--config", default="ns14_roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-solves", action="store_true", help="skip the 14-site solves and the 15-site DMFT iteration")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--strip", type=int, default=0)
     ap.add_argument("--copy-check", type=int, default=0,
@@ -433,6 +466,7 @@ def main():
     gbs = alg_bytes / (ms * 1e-3) / 1e9
     traffic = hbm_traffic(os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.md"))
     batched = None if args.no_batched else batched_vs_plain()
+    solves = None if (args.no_batched or args.no_solves) else config_solves()
     cpu = cpu_extra = None
     if not args.no_cpu:
         cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
@@ -451,6 +485,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_more": cpu_extra,
         "batched_vs_plain": batched,
+        "config_solves": solves,
         "wall_s_warm_and_timed": round(wall, 4),
         "untimed_load_steps": n_warm * args.steps,
         "ms_per_step_eager": round(ms_eager, 6),
