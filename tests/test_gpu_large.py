@@ -60,6 +60,33 @@ def test_ns10_all_sectors_eigenvalues_states_and_green(flags):
     assert e <= 1e-8
 
 
+def test_ns10_batched_solve_is_deterministic_and_the_check_inserts_nothing_here():
+    """The batched solve's convergence tests run on their own stream and enter the stop flags in
+    stream order, and the persistent launch hands its sectors to whichever workgroup is free: the
+    results must still be the same on every run (same handle, fresh handle).  This model has no
+    degenerate level among any sector's 8 lowest, so the degeneracy check inserts nothing and the
+    lists and G equal those of a run without it."""
+    z = np.load(os.path.join(GOLD, "ns10_batched.npz"))
+    c = config("ns10_batched")
+    om = z["omega"]
+    nsec = len(c.sectors)
+
+    def run(s):
+        s.ed_solve()
+        ev = np.array([s.sector_eigenvalues(i + 1) for i in range(nsec)])
+        return ev, s.ed_calc_green_ftn(om)
+
+    with EDSolver.from_config(c, flags=moed.MOED_F_BATCHED) as s:
+        runs = [run(s), run(s)]
+    with EDSolver.from_config(c, flags=moed.MOED_F_BATCHED) as s:
+        runs.append(run(s))
+    for ev, G in runs[1:]:
+        assert np.array_equal(ev, runs[0][0]) and np.array_equal(G, runs[0][1])
+    with EDSolver.from_config(c, flags=moed.MOED_F_BATCHED | moed.MOED_F_NO_DEGENERACY_CHECK) as s:
+        ev, G = run(s)
+    assert np.array_equal(ev, runs[0][0]) and np.array_equal(G, runs[0][1])
+
+
 def test_ns15_shape_fit_reference_order_bit_exact():
     z = np.load(os.path.join(GOLD, "ns15_fit.npz"))
     norb, nbath = int(z["norb"]), int(z["nbath"])
