@@ -87,6 +87,24 @@ def test_ns10_batched_solve_is_deterministic_and_the_check_inserts_nothing_here(
     assert np.array_equal(ev, runs[0][0]) and np.array_equal(G, runs[0][1])
 
 
+def test_ns14_ground_state_and_green_against_oracle():
+    z = np.load(os.path.join(GOLD, "ns14_green.npz"))
+    c = config("ns14_roofline")
+    print("ns14_roofline: sector (7, 7), D = 11778624, 2048 frequencies", flush=True)
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        G = s.ed_calc_green_ftn(z["omega"])
+        recs = s.green_records()
+    E0 = float(z["E0"])
+    print(f"  E0 gpu {kept[0][0]!r} arpack {E0!r}", flush=True)
+    assert abs(kept[0][0] - E0) <= 1e-10 * abs(E0)
+    for r in recs:  # particle weights ||c+ psi||^2 of the oracle's chains
+        assert abs(r["bp"][0] - float(z[f"bp0_{r['iorb']}"])) <= 1e-10
+    e = _gerr(G, z["G"])
+    print(f"  G: rel err {e:.2e}", flush=True)
+    assert e <= 1e-8
+
+
 def test_ns15_shape_fit_reference_order_bit_exact():
     z = np.load(os.path.join(GOLD, "ns15_fit.npz"))
     norb, nbath = int(z["norb"]), int(z["nbath"])
