@@ -593,39 +593,6 @@ int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep
 
 // ----------------------------------------------------------------- eigensolver (replaces PARPACK diag)
 
-int eig_dense(moed_handle* h, const Sector& s, int nev, SectorEig& E) {
-    const int D = (int)s.D;
-    hipStream_t st = h->stream;
-    DevBuf& X = h->ws[WS_X];
-    DevBuf& Y = h->ws[WS_Y];
-    HIPCHK(X.ensure((size_t)D * D * sizeof(double)));
-    HIPCHK(Y.ensure((size_t)D * D * sizeof(double)));
-    std::vector<double> I((size_t)D * D, 0.0), Hm((size_t)D * D);
-    for (int j = 0; j < D; ++j) I[(size_t)j * D + j] = 1.0;
-    HIPCHK(hipMemcpyAsync(X.p, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    for (int j = 0; j < D; ++j) {
-        int rc = hv_dev(h, s, X.as<double>() + (size_t)j * D, Y.as<double>() + (size_t)j * D, nullptr, nullptr,
-                        nullptr, st);
-        if (rc) return rc;
-    }
-    HIPCHK(hipMemcpyAsync(Hm.data(), Y.p, Hm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    // column j of H is Hm[j*D : (j+1)*D]; H is symmetric, use it as row-major
-    std::vector<double> w, Vm;
-    jacobi_eig(Hm, D, w, Vm);
-    const int nf = std::min(nev, D);
-    E.nfound = nf;
-    E.ev.assign(nev, DBL_MAX);
-    std::vector<double> vecs((size_t)nf * D);
-    for (int i = 0; i < nf; ++i) {
-        E.ev[i] = w[i];
-        for (int r = 0; r < D; ++r) vecs[(size_t)i * D + r] = Vm[(size_t)r * D + i];
-    }
-    HIPCHK(E.vec.ensure(vecs.size() * sizeof(double)));
-    HIPCHK(hipMemcpy(E.vec.p, vecs.data(), vecs.size() * sizeof(double), hipMemcpyHostToDevice));
-    return MOED_OK;
-}
-
 // Lanczos with partial reorthogonalisation for several sectors at once; one sector is the
 // per-sector form of the same code.  Every problem p runs the same iteration as alone: start
 // vector u_0 (counter hash, seed of its sector), then per step w = H u_k / |u_k|, u_{k+1} = w minus
@@ -1156,8 +1123,8 @@ int degeneracy_check(moed_handle* h, int nev, std::vector<int> cand) {
 
 // Every sector of at most 64 states of a solve at once: their matrices from one batched product
 // (each column of each such sector a problem of hv_batch_kernel applied to a unit vector; its terms
-// per element are those of the two-pass product, so H is the one eig_dense builds), then the dense
-// eigensolver of eig_dense on the host, the sectors in parallel.
+// per element are those of the two-pass product), then cyclic Jacobi on the host, the sectors in
+// parallel.
 int eig_dense_batch(moed_handle* h, const std::vector<int>& secs, int nev) {
     if (secs.empty()) return MOED_OK;
     hipStream_t st = h->stream;
@@ -1255,7 +1222,6 @@ int eig_sector(moed_handle* h, int nu, int nd, int nev, SectorEig& E) {
     E.nu = nu;
     E.nd = nd;
     E.D = s.D;
-    if (s.D <= 64) return eig_dense(h, s, nev, E);
     std::vector<SectorEig*> out{&E};
     return eig_batch(h, {{nu, nd}}, nev, out);
 }
