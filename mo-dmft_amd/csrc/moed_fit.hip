@@ -40,6 +40,7 @@ constexpr int kFitMaxBath = kMaxSite - 1;
 constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
 constexpr int kFitChunkF = kFitThreads / 4 * 3;  // frequencies per staged FUNC chunk (waves w % 4 != 0)
 constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
+constexpr int kGrp = 4;                        // tree order: frequencies per thread and group
 constexpr int kItmax = 20000, kBrentItmax = 100;  // frprmn.f90:12, :126
 constexpr int kStateD = 3 * kFitNx + 2;       // per fit: xi, g, h, fp, fret
 constexpr int kStateI = 4;                    // per fit: phase, its, rc, nfev
@@ -78,16 +79,26 @@ struct FitSmem {
     // tree order: the nonzero couplings of each orbital at the point xt (V^2, V, e_k, k)
     double nzv2[kMaxOrb][kFitMaxBath], nzv[kMaxOrb][kFitMaxBath], nzek[kMaxOrb][kFitMaxBath];
     int32_t nzk[kMaxOrb][kFitMaxBath], nnz[kMaxOrb];
-    union {
+    union alignas(16) {  // 16-byte aligned: chain_sum reads the staged terms in pairs
         double fstage[2][kFitChunkF * kMaxOrb];  // FUNC terms, two chunks (reference order)
         double cols[kFitCols * kFitChunkG];      // gradient terms of one chunk (reference order)
         struct {  // tree order, Nw <= kFitThreads: thread t's frequency iw = t, set once per launch
             double zi[kFitThreads], rw[kFitThreads];                  // Im z, 1/w_n
             double dre[kMaxOrb][kFitThreads], dim[kMaxOrb][kFitThreads];  // the target Dev_o(w)
         } tc;
+        struct {  // tree order, kFitThreads < Nw <= kGrp kFitThreads: iw = t + j kFitThreads
+            double zi[kGrp * kFitThreads], rw[kGrp * kFitThreads];  // Im z, 1/w_n (0 past the end)
+        } tw;
     } u;
-    double fred[kFitThreads / 64];  // tree-order FUNC: the waves' partial sums
-    int32_t tc_ok;                  // u.tc holds the frequencies of this launch
+    // tree order, per line search: the couplings that can be nonzero on the line pcom + t xicom
+    // (V(o,k) with pcom or xicom nonzero; any other V is exactly 0 at every finite t)
+    int32_t lk[kMaxOrb][16], lnn[kMaxOrb];
+    // tree order, per wave: the couplings of the point being evaluated, formed by the wave itself
+    struct {
+        double ek[kMaxOrb][16], v2[kMaxOrb][16], ef[kMaxOrb];
+    } wp[kFitThreads / 64];
+    double fred[2][kFitThreads / 64];  // tree-order FUNC: the waves' partial sums (two buffers)
+    int32_t tc_mode;                   // 1: u.tc, 2: u.tw holds the frequencies of this launch
 };
 
 // the fit's LDS, shared by every device function of this file (one workgroup = one fit)
@@ -130,28 +141,45 @@ __device__ __forceinline__ cxd residual(const FitView& q, const double* xt, int 
     return {(re + xt[q.nx - q.norb + o]) - q.dev[2 * io], im - q.dev[2 * io + 1]};
 }
 
-// s + buf[0] + buf[1] + ... + buf[n-1], one rounding per add in this order; the LDS loads run
-// sixteen elements ahead of the dependent adds
+// s + buf[0] + buf[1] + ... + buf[n-1], one rounding per add in this order.  buf (LDS) is 16-byte
+// aligned.  The terms arrive as pairs through a ring of kChainPairs 16-byte loads: each slot is
+// reloaded right after its pair is added, so the oldest load is the only one an add waits for and
+// the dependent adds run at the adder's latency (~7 cycles on gfx950) instead of waiting for a whole
+// batch of loads.
+constexpr int kChainPairs = 16;
 __device__ __forceinline__ double chain_sum(double s, const double* buf, int n) {
 #pragma clang fp contract(off)
-    constexpr int B = 16;
-    int j = 0;
-    if (n >= 2 * B) {
-        double a[B], b[B];
+    const double2* b2 = reinterpret_cast<const double2*>(buf);
+    const int n2 = n >> 1;
+    int j = 0;  // pairs loaded
+    if (n2 >= 2 * kChainPairs) {
+        double2 r[kChainPairs];
 #pragma unroll
-        for (int t = 0; t < B; ++t) a[t] = buf[t];
-        for (j = B; j + B <= n; j += B) {
+        for (int t = 0; t < kChainPairs; ++t) {
+            r[t] = b2[t];
+            __builtin_amdgcn_sched_barrier(0);  // issued in the order the loop consumes them
+        }
+        for (j = kChainPairs; j + kChainPairs <= n2; j += kChainPairs) {
 #pragma unroll
-            for (int t = 0; t < B; ++t) b[t] = buf[j + t];
-#pragma unroll
-            for (int t = 0; t < B; ++t) s = s + a[t];
-#pragma unroll
-            for (int t = 0; t < B; ++t) a[t] = b[t];
+            for (int t = 0; t < kChainPairs; ++t) {
+                s = s + r[t].x;
+                s = s + r[t].y;
+                r[t] = b2[j + t];
+                __builtin_amdgcn_sched_barrier(0);  // keep the ring's order (one load per pair)
+            }
         }
 #pragma unroll
-        for (int t = 0; t < B; ++t) s = s + a[t];
+        for (int t = 0; t < kChainPairs; ++t) {
+            s = s + r[t].x;
+            s = s + r[t].y;
+        }
     }
-    for (; j < n; ++j) s = s + buf[j];
+    for (; j < n2; ++j) {
+        const double2 v = b2[j];
+        s = s + v.x;
+        s = s + v.y;
+    }
+    if (n & 1) s = s + buf[n - 1];
     return s;
 }
 
@@ -204,7 +232,6 @@ __device__ __forceinline__ void nonzero_couplings(const FitView& q) {
 // for every thread: the group helpers are instantiated per slot count JN = jn and skip them.  Their
 // terms are exactly +0.0 (zero residual, zero weight), so skipping them leaves every sum
 // bit-identical while a short frequency axis (Nw < kGrp kFitThreads) costs only the slots it fills.
-constexpr int kGrp = 4;
 struct FreqGroup {
     double zi[kGrp], rw[kGrp];  // Im z, 1/w_n (0 for a frequency past the end)
     int iw[kGrp];
@@ -336,13 +363,16 @@ __device__ __noinline__ double eval_func_ref(const FitProb qa) {
     return S.bcast[0];
 }
 
-// Tree order with the frequencies cached in LDS (Nw <= kFitThreads): thread t's frequency iw = t.
-// Fill once per launch; ends with a barrier.
+// Tree order with the frequencies cached in LDS, filled once per launch; ends with a barrier.
+// Mode 1 (Nw <= kFitThreads): Im z, 1/w_n and the target of thread t's frequency iw = t.
+// Mode 2 (Nw <= kGrp kFitThreads): Im z and 1/w_n of every slot iw = t + j kFitThreads, with
+// freq_group's values for a slot past the end (Im z of w = 1, weight 0); the target stays in HBM.
 __device__ void tree_cache_fill(const FitProb qa) {
     const FitView q = view(qa);
     const int t = threadIdx.x;
-    const bool on = uniform_i32(qa.tree) && q.nw <= kFitThreads;
-    if (on) {
+    const bool tree = uniform_i32(qa.tree);
+    const int mode = !tree ? 0 : q.nw <= kFitThreads ? 1 : q.nw <= kGrp * kFitThreads ? 2 : 0;
+    if (mode == 1) {
         const bool ok = t < q.nw;
         const double w = ok ? q.omega[t] : 1.0;
         S.u.tc.zi[t] = zim(q, w);
@@ -352,8 +382,16 @@ __device__ void tree_cache_fill(const FitProb qa) {
             S.u.tc.dre[o][t] = q.dev[2 * io];
             S.u.tc.dim[o][t] = q.dev[2 * io + 1];
         }
+    } else if (mode == 2) {
+        for (int j = 0; j < kGrp; ++j) {
+            const int iw = t + j * kFitThreads;
+            const bool ok = iw < q.nw;
+            const double w = ok ? q.omega[iw] : 1.0;
+            S.u.tw.zi[iw] = zim(q, w);
+            S.u.tw.rw[iw] = ok ? 1.0 / w : 0.0;
+        }
     }
-    if (t == 0) S.tc_ok = on;
+    if (t == 0) S.tc_mode = mode;
     __syncthreads();
 }
 
@@ -377,10 +415,10 @@ __device__ __forceinline__ void nonzero_couplings_wave0(const FitView& q) {
     if (k == 0 && o < q.norb) S.nnz[o] = __popcll(seg);
 }
 
-// FUNC in tree order from the cached frequencies and the coupling lists already built for xt;
-// the same operations as eval_func_tree with one frequency per thread, one barrier: every thread
-// adds the waves' partials in index order (block_sum's order).  S.fred is written again only by
-// the next evaluation, after the barrier of its point update.
+// FUNC in tree order from the cached frequencies (mode 1) and the coupling lists already built
+// for xt; the same operations as eval_func_tree with one frequency per thread, one barrier: every
+// thread adds the waves' partials in index order (block_sum's order).  Callers are preceded by a
+// barrier, so no wave still reads the partials buffer of an earlier evaluation.
 __device__ __forceinline__ double eval_func_tree_cached(const FitProb& qa) {
 #pragma clang fp contract(off)
     const FitView q = view(qa);
@@ -403,20 +441,137 @@ __device__ __forceinline__ double eval_func_tree_cached(const FitProb& qa) {
         }
     }
     s = wave_sum(s);
-    if ((tid & 63) == 0) S.fred[tid >> 6] = s;
+    if ((tid & 63) == 0) S.fred[0][tid >> 6] = s;
     __syncthreads();
     double tot = 0.0;
-    for (int w = 0; w < kFitThreads / 64; ++w) tot += S.fred[w];
+    for (int w = 0; w < kFitThreads / 64; ++w) tot += S.fred[0][w];
     return tot / (double)(float)(q.norb * q.norb * q.nw);
 }
 
 __device__ __forceinline__ double eval_func(const FitProb& q) {
-    if (S.tc_ok) {  // block-uniform
+    if (uniform_i32(S.tc_mode) == 1) {
         if (threadIdx.x < 64) nonzero_couplings_wave0(view(q));
         __syncthreads();
         return eval_func_tree_cached(q);
     }
     return uniform_i32(q.tree) ? eval_func_tree(q) : eval_func_ref(q);
+}
+
+// The couplings that can be nonzero on the line pcom + t xicom (wave 0, in k order, as
+// nonzero_couplings orders them); ends with a barrier.  A coupling on the list that is exactly zero
+// at some t adds fma(0, d, r) = r to each sum (r starts at +0.0 and d is finite), so evaluations
+// over this list equal evaluations over the point's own nonzero list bit for bit.
+__device__ __forceinline__ void line_couplings(const FitView& q) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int o = tid >> 4, k = tid & 15;
+        const bool in = o < q.norb && k < q.nbath;
+        const int j = in ? q.nbath + o * q.nbath + k : 0;
+        const bool nz = in && (S.pcom[j] != 0.0 || S.xicom[j] != 0.0);
+        const uint64_t mask = __ballot(nz);
+        const uint64_t seg = mask & (0xffffull << (16 * o));
+        if (nz) S.lk[o][__popcll(seg & ((1ull << tid) - 1ull))] = k;
+        if (k == 0 && o < q.norb) S.lnn[o] = __popcll(seg);
+    }
+    __syncthreads();
+}
+
+// mode 2: the terms of thread t's JN frequency slots, func_group's operations and order
+template <int JN>
+__device__ __forceinline__ double func_point(const FitView& q, const double (*ek)[16], const double (*v2)[16],
+                                             const double* ef) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    double zi[JN], rw[JN];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+        zi[j] = S.u.tw.zi[tid + j * kFitThreads];
+        rw[j] = S.u.tw.rw[tid + j * kFitThreads];
+    }
+    double s = 0.0;
+    for (int o = 0; o < q.norb; ++o) {
+        double dre[JN], dim[JN], re[JN], im[JN];
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int iw = tid + j * kFitThreads;
+            const size_t io = (size_t)o + (size_t)q.norb * (iw < q.nw ? iw : 0);
+            dre[j] = q.dev[2 * io];
+            dim[j] = q.dev[2 * io + 1];
+            re[j] = im[j] = 0.0;
+        }
+        const int nn = uniform_i32(S.lnn[o]);
+        for (int n = 0; n < nn; ++n) {
+            const double e = ek[o][n], w2 = v2[o][n];
+#pragma unroll
+            for (int j = 0; j < JN; ++j) {
+                const cxd d = inv_den(e, zi[j]);
+                re[j] = fma(w2, d.re, re[j]);
+                im[j] = fma(w2, d.im, im[j]);
+            }
+        }
+        const double f = ef[o];
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const bool ok = tid + j * kFitThreads < q.nw;
+            const double r1 = ok ? (re[j] + f) - dre[j] : 0.0, r2 = ok ? im[j] - dim[j] : 0.0;
+            s = fma(fma(r1, r1, r2 * r2), rw[j], s);
+        }
+    }
+    return s;
+}
+
+// f1dim's FUNC in tree order with cached frequencies (mode 2) at pcom + xv xicom.  Every wave forms the
+// point's couplings itself in its own LDS slot (lane 16 o + n: coupling n of orbital o; lane
+// 16 o + 15: ef_o), with f1dim's arithmetic, so no block barrier precedes the sums; the one barrier
+// is the waves' partials, in buffer `buf`.  Consecutive calls alternate buffers: a wave can reach
+// the next evaluation's write only after every wave has left this one's barrier, and it can reach
+// the write after that only after every wave has read this one's partials.  Equal bit for bit to
+// writing xt and calling eval_func.
+__device__ __forceinline__ void form_point(const FitView& q, double xv) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    auto& P = S.wp[threadIdx.x >> 6];
+    const int o = lane >> 4, n = lane & 15;
+    if (o < q.norb) {
+        if (n < S.lnn[o]) {
+            const int k = S.lk[o][n], j = q.nbath + o * q.nbath + k;
+            const double v = S.pcom[j] + xv * S.xicom[j];
+            P.ek[o][n] = S.pcom[k] + xv * S.xicom[k];
+            P.v2[o][n] = v * v;
+        } else if (n == 15) {
+            const int j = q.nx - q.norb + o;
+            P.ef[o] = S.pcom[j] + xv * S.xicom[j];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the waves' partials of one evaluation, summed in block_sum's order by every thread
+__device__ __forceinline__ double point_total(const FitView& q, double s, int buf) {
+#pragma clang fp contract(off)
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) S.fred[buf][threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < kFitThreads / 64; ++w) tot += S.fred[buf][w];
+    return tot / (double)(float)(q.norb * q.norb * q.nw);
+}
+
+// mode 2: up to kGrp frequencies per thread (eval_func_tree's operations); out of line, so the
+// line search keeps one copy of it
+__device__ __noinline__ double f1dim_tree_wide(const FitProb qa, double xv, int buf) {
+    const FitView q = view(qa);
+    form_point(q, xv);
+    const auto& P = S.wp[threadIdx.x >> 6];
+    double s;
+    switch ((q.nw + kFitThreads - 1) / kFitThreads) {
+        case 4: s = func_point<4>(q, P.ek, P.v2, P.ef); break;
+        case 3: s = func_point<3>(q, P.ek, P.v2, P.ef); break;
+        default: s = func_point<2>(q, P.ek, P.v2, P.ef); break;
+    }
+    return point_total(q, s, buf);
 }
 
 // gradient terms of one (w, o) (func.f90:89-124): column k = d/de_k, nbath + k = d/dV(o,k),
@@ -560,7 +715,10 @@ __device__ __forceinline__ void eval_grad(const FitProb& q, double* out) {
 __device__ __forceinline__ double f1dim(const FitProb& q, double xv, int& nfev) {
 #pragma clang fp contract(off)
     ++nfev;
-    if (S.tc_ok) {  // nx <= 64: wave 0 writes the point and builds its coupling lists, one barrier
+    // tree order with cached frequencies: each wave forms the point itself (a non-finite xv takes
+    // the path below, where every coupling is formed from xt)
+    const int mode = uniform_i32(S.tc_mode);
+    if (mode == 1) {  // wave 0 writes the point and builds its coupling lists, one barrier
         if (threadIdx.x < 64) {
             const int j = threadIdx.x;
             if (j < q.nx) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
@@ -572,6 +730,9 @@ __device__ __forceinline__ double f1dim(const FitProb& q, double xv, int& nfev) 
         __syncthreads();
         return eval_func_tree_cached(q);
     }
+    // up to kGrp frequencies per thread: each wave forms the point itself (a non-finite xv takes
+    // the path below, where every coupling is formed from xt)
+    if (mode == 2 && isfinite(xv)) return f1dim_tree_wide(q, xv, nfev & 1);
     for (int j = threadIdx.x; j < q.nx; j += kFitThreads) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
     __syncthreads();
     return eval_func(q);
@@ -767,6 +928,7 @@ __global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
             S.xicom[j] = S.xi[j];
         }
         __syncthreads();
+        if (uniform_i32(S.tc_mode) == 2) line_couplings(view(q));
         double ax = 0., xx = 1., bx, fa, fx, fb, xmin = 0.0, fr = 0.0;
         if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
             rc = MOED_ECONV;  // 'brent exceed maximum iterations'
