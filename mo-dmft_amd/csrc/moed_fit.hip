@@ -142,11 +142,12 @@ __device__ __forceinline__ cxd residual(const FitView& q, const double* xt, int 
 }
 
 // s + buf[0] + buf[1] + ... + buf[n-1], one rounding per add in this order.  buf (LDS) is 16-byte
-// aligned.  The terms arrive as pairs through a ring of kChainPairs 16-byte loads: each slot is
-// reloaded right after its pair is added, so the oldest load is the only one an add waits for and
-// the dependent adds run at the adder's latency (~7 cycles on gfx950) instead of waiting for a whole
-// batch of loads.
-constexpr int kChainPairs = 16;
+// aligned.  The terms arrive as pairs through a ring of kChainPairs 16-byte loads, refilled
+// kChainGroup loads at a time right after their pairs are added: an add waits only for loads
+// issued a whole ring earlier.  Measured on gfx950 with one lane active: a dependent FP64 add takes
+// 8.4 cycles and each LDS load instruction holds its wave ~8 cycles, so an LDS-fed chain costs
+// ~12.6 cycles per add with loads in groups of four (15.3 with single loads between adds).
+constexpr int kChainPairs = 16, kChainGroup = 4;
 __device__ __forceinline__ double chain_sum(double s, const double* buf, int n) {
 #pragma clang fp contract(off)
     const double2* b2 = reinterpret_cast<const double2*>(buf);
@@ -155,17 +156,20 @@ __device__ __forceinline__ double chain_sum(double s, const double* buf, int n) 
     if (n2 >= 2 * kChainPairs) {
         double2 r[kChainPairs];
 #pragma unroll
-        for (int t = 0; t < kChainPairs; ++t) {
-            r[t] = b2[t];
-            __builtin_amdgcn_sched_barrier(0);  // issued in the order the loop consumes them
-        }
+        for (int t = 0; t < kChainPairs; ++t) r[t] = b2[t];
+        __builtin_amdgcn_sched_barrier(0);
         for (j = kChainPairs; j + kChainPairs <= n2; j += kChainPairs) {
 #pragma unroll
-            for (int t = 0; t < kChainPairs; ++t) {
-                s = s + r[t].x;
-                s = s + r[t].y;
-                r[t] = b2[j + t];
-                __builtin_amdgcn_sched_barrier(0);  // keep the ring's order (one load per pair)
+            for (int g = 0; g < kChainPairs; g += kChainGroup) {
+#pragma unroll
+                for (int t = g; t < g + kChainGroup; ++t) {
+                    s = s + r[t].x;
+                    s = s + r[t].y;
+                }
+                __builtin_amdgcn_sched_barrier(0);  // keep the ring's order
+#pragma unroll
+                for (int t = g; t < g + kChainGroup; ++t) r[t] = b2[j + t];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
 #pragma unroll
