@@ -8,6 +8,11 @@ the host cores over MPI ranks as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--config ns14_roofline]
 
+The same line also carries the other workloads' wall times, each timed in the same run: the
+10-site batched-versus-plain solve (with one DMFT iteration's lattice step and bath fits), the
+14-site solves, and the 8-iteration DMFT loop at the 15-site fit workload in both fit orders
+(about four minutes; --no-dmft skips the loops, --no-solves the loops and the 14-site solves).
+
 With --dmft it instead runs the DMFT self-consistency loop of main.f90 on the 15-site, three-orbital
 sector (7, 8) (D = 41,409,225), 4096 frequencies at beta = 200, the tight-binding lattice step over
 Nq = 10^4 k-points and --nfit bath fits per iteration in one launch, and prints one JSON line with
@@ -179,12 +184,10 @@ def gpu_extra_timings():
 
 
 def config_solves():
-    """wall times of the other configurations' own work, timed here so that the line carries them:
-    the 14-site sector's solve and Green's functions (partial reorthogonalisation, and the solve
-    with full reorthogonalisation on the FP64 matrix cores), and one iteration of the DMFT loop at
-    the 15-site fit workload (solve, G, lattice step, 256 tree-order fits in one launch)"""
+    """wall times of the 14-site sector's solve and Green's functions, timed here so that the line
+    carries them: partial reorthogonalisation, and the solve with full reorthogonalisation on the
+    FP64 matrix cores"""
     import moed
-    from moed import dmft
     from moed.model import config, omega_grid
     out = {}
     c3 = config("ns14_roofline")
@@ -197,17 +200,59 @@ def config_solves():
             s.ed_calc_green_ftn(om)
             t2 = time.perf_counter()
         out[name] = {"t_solve_s": round(t1 - t0, 4), "t_green_s": round(t2 - t1, 4), "E0": kept[0][0]}
+    return out
+
+
+def dmft_loops(niter=8, nfit=256):
+    """the DMFT loop (main.f90:59-127) at the 15-site fit workload, niter iterations of solve, G,
+    lattice step and nfit bath fits in one launch, once with each fit order: reference order (every
+    fit's CG trajectory bit-identical to the reference's minimization for its inputs) and tree
+    order (faster sums, rounded differently); progress per iteration on stderr"""
+    import numpy as np
+    import moed
+    from moed import dmft
+    from moed.model import config, omega_grid
     c5 = config("ns15_fit")
     om5 = omega_grid(c5.nw, c5.beta)
     hk = moed.band_tb(c5.norb, 10000)
-    with moed.EDSolver.from_config(c5, flags=moed.MOED_F_FIT_TREE) as s:
-        recs, _ = dmft.dmft_loop(s, hk, c5.mu, om5, 1, nfit=256, scf_tol=0.0)
-    r = recs[0]
-    out["ns15_one_dmft_iteration"] = {k: round(float(r[k]), 4) for k in ("t_solve", "t_green", "t_delta", "t_fit",
-                                                                          "t_total")}
-    out["ns15_one_dmft_iteration"].update({"E0": float(r["E0"]), "chi2": float(r["fret"]), "nfit": 256,
-                                           "fit_order": "tree"})
-    return out
+    keys = ("t_solve", "t_green", "t_delta", "t_fit", "t_total")
+    out, baths = {}, {}
+    for order, flags in (("reference", 0), ("tree", moed.MOED_F_FIT_TREE)):
+        def report(r):
+            print(f"bench.py: DMFT loop ({order} order) iteration {r['iteration']}: {r['t_total']:.2f} s "
+                  f"(fit {r['t_fit']:.2f}), chi2 {r['fret']:.4e}", file=sys.stderr, flush=True)
+
+        with moed.EDSolver.from_config(c5, flags=flags) as s:
+            recs, bath = dmft.dmft_loop(s, hk, c5.mu, om5, niter, nfit=nfit, scf_tol=0.0, on_iter=report)
+        baths[order] = bath
+        out[order] = {"wall_s": round(sum(r["t_total"] for r in recs), 3),
+                      "per_stage_s": {k: round(sum(r[k] for r in recs), 3) for k in keys},
+                      "fit_s_per_iteration": [round(r["t_fit"], 3) for r in recs],
+                      "final_chi2": float(f"{recs[-1]['fret']:.6g}"),
+                      "final_scf_diff": float(f"{recs[-1]['scf_diff']:.4g}")}
+    # The fitted hybridisation depends on V(o,k)^2 and not on the order of an orbital's bath sites,
+    # so the two orders' final baths are compared in a canonical form (per orbital its sites sorted
+    # by energy, |V|, then ef) and through Delta_o(i w_n) itself.
+    def canonical(b):
+        ef, ek, vk = b
+        parts = []
+        for o in range(c5.norb):
+            ks = np.nonzero(vk[o])[0]
+            parts.append(np.array(sorted(zip(ek[c5.norb + ks], np.abs(vk[o, ks])))).ravel())
+        return np.concatenate(parts + [ef])
+
+    def delta(b):
+        ef, ek, vk = b
+        z = 1j * om5[None, :, None] - ek[c5.norb:][None, None, :]
+        return np.sum(vk[:, None, :] ** 2 / z, axis=2)
+
+    dt, dr = delta(baths["tree"]), delta(baths["reference"])
+    return {"workload": f"DMFT loop, {c5.nsite}-site sector (7, 8), D = 41409225, t_ij on, {c5.nw} frequencies, "
+                        f"Nq = 10^4, {nfit} bath fits per iteration in one launch, {niter} iterations",
+            "iterations": niter, "nfit": nfit, **out,
+            "final_bath_tree_vs_reference": {
+                "canonical_max_abs_diff": float(f"{np.max(np.abs(canonical(baths['tree']) - canonical(baths['reference']))):.3g}"),
+                "delta_max_rel_diff": float(f"{np.max(np.abs(dt - dr)) / np.max(np.abs(dr)):.3g}")}}
 
 
 def config_of(name):
@@ -364,7 +409,9 @@ def main():
     ap.add_argument("--config", default="ns14_roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-solves", action="store_true", help="skip the 14-site solves and the 15-site DMFT iteration")
+    ap.add_argument("--no-solves", action="store_true",
+                    help="skip the 14-site solves and the 15-site DMFT loops")
+    ap.add_argument("--no-dmft", action="store_true", help="skip the 15-site DMFT loops (both fit orders)")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--strip", type=int, default=0)
     ap.add_argument("--copy-check", type=int, default=0,
@@ -467,6 +514,7 @@ def main():
     traffic = hbm_traffic(os.path.join(ROOT, "profiles", f"hv_{args.config}_traffic.md"))
     batched = None if args.no_batched else batched_vs_plain()
     solves = None if (args.no_batched or args.no_solves) else config_solves()
+    loops = None if (args.no_batched or args.no_solves or args.no_dmft) else dmft_loops()
     cpu = cpu_extra = None
     if not args.no_cpu:
         cpu = cpu_reference_hv(cfg, nu, nd) or cpu_port_hv(cfg, nu, nd)
@@ -486,6 +534,7 @@ def main():
         "cpu_baseline_more": cpu_extra,
         "batched_vs_plain": batched,
         "config_solves": solves,
+        "dmft_loop": loops,
         "wall_s_warm_and_timed": round(wall, 4),
         "untimed_load_steps": n_warm * args.steps,
         "ms_per_step_eager": round(ms_eager, 6),
