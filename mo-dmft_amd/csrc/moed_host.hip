@@ -506,8 +506,9 @@ int lanczos_ref_dev(moed_handle* h, const Sector& s, const double* u0, int nstep
     // rows of odd length run padded to an even stride, so that the H v strip pass reads 16-byte
     // pairs; the padding columns hold zeros throughout (H v never writes them and the three-term
     // updates of zeros stay zero), so every sum is over the sector's own elements.  With three or
-    // more orbitals the two-column strip kernel holds too many registers (153 VGPRs, 3 waves per
-    // SIMD) and is slower than the one-column kernel on the unpadded rows, so those stay as they are.
+    // more orbitals the two-column strip kernel (85 VGPRs, 5 waves per SIMD) is slower than the
+    // one-column kernel on the unpadded rows (15-site G stage 1.55 against 1.51 s), so those stay
+    // as they are.
     const int64_t nU = s.up->count, nD = s.dn->count;
     const int64_t ldx = (nU % 2 == 1 && h->hv_strip != 64 && h->norb <= 2) ? nU + 1 : nU;
     const int64_t D = nD * ldx;

@@ -181,6 +181,7 @@ template <int NORB, int CPL>
 __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
     constexpr int NW = kStripThreads / 64;
     constexpr int NP = NORB * NORB;
+    constexpr int MU = NORB * NORB / 4;  // most ordered impurity pairs one up string can move
     constexpr int W = 64 * CPL;
     __shared__ double s_red[NW];
 
@@ -211,14 +212,30 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
         const bool ok1 = CPL == 1 || c00 + 1 < nU;
         const int iu = ok ? c00 : ((nU - CPL) & ~(CPL - 1));
         const int r1 = min(nD, (rb + 1) * P.rows_b);
-        // up half of the spin-flip / pair-exchange terms: move j -> i of each column's up string
-        int32_t mu[NP > 1 ? NP : 1][CPL];
+        // up half of the spin-flip / pair-exchange terms, compacted per column: pair p = (i, j) needs
+        // up j occupied and up i empty (move j -> i), which holds for at most MU of the ordered pairs
+        // of one up string (k (NORB - k) <= NORB^2 / 4 with k impurity electrons).  Slot s of column
+        // c keeps the s-th applicable pair in ascending order as (p << 24) | target | sign << 16, or
+        // -1: the terms are then summed in ascending pair order, as over the full pair list.
+        int32_t ua[MU > 0 ? MU : 1][CPL];
         if constexpr (NORB > 1) {
 #pragma unroll
-            for (int p = 0; p < NP; ++p)
+            for (int c = 0; c < CPL; ++c) {
 #pragma unroll
-                for (int c = 0; c < CPL; ++c)
-                    mu[p][c] = (p / NORB != p % NORB && (c == 0 || ok1)) ? P.up.imp[(size_t)p * nU + iu + c] : -1;
+                for (int k = 0; k < MU; ++k) ua[k][c] = -1;
+                int n = 0;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    if (p / NORB == p % NORB) continue;
+                    const int32_t v = (c == 0 || ok1) ? P.up.imp[(size_t)p * nU + iu + c] : -1;
+                    if (v >= 0) {
+#pragma unroll
+                        for (int k = 0; k < MU; ++k)
+                            if (n == k) ua[k][c] = (p << 24) | v;
+                        ++n;
+                    }
+                }
+            }
         }
         for (int id = rb * P.rows_b + wave; id < r1; id += NW) {
             const size_t ro = (size_t)id * P.ldx;
@@ -232,13 +249,20 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 t[q] = mvp[m0 + q];
                 cf[q] = mvcp[m0 + q];
             }
-            int32_t msf[NP > 1 ? NP : 1], mpe[NP > 1 ? NP : 1];
+            // down half per pair p = (i, j): spin flip moves down i -> j, pair exchange down j -> i; one
+            // row has at most one of them (down i occupied and j empty, or the reverse), and both give
+            // the element +-J times the two signs, so the row keeps one entry per pair
+            int32_t dh[NP > 1 ? NP : 1];
             if constexpr (NORB > 1) {
 #pragma unroll
                 for (int p = 0; p < NP; ++p) {
                     const int i = p / NORB, j = p % NORB;
-                    msf[p] = i != j ? dimpc[(size_t)(j * NORB + i) * nD + id] : -1;
-                    mpe[p] = i != j ? dimpc[(size_t)(i * NORB + j) * nD + id] : -1;
+                    if (i == j) {
+                        dh[p] = -1;
+                        continue;
+                    }
+                    const int32_t sf = dimpc[(size_t)(j * NORB + i) * nD + id];
+                    dh[p] = sf >= 0 ? sf : dimpc[(size_t)(i * NORB + j) * nD + id];
                 }
             }
             // ---- every vector load of the row before any use
@@ -249,18 +273,23 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
                 if (q < nh) xv[q] = load_cols<CPL>(X + (size_t)t[q] * P.ldx + iu);
-            double gs[NP > 1 ? NP : 1][CPL], gp[NP > 1 ? NP : 1][CPL];
+            // each column's applicable spin-flip / pair-exchange entries: the row's entry of the slot's pair
+            int32_t dsel[MU > 0 ? MU : 1][CPL];
+            double gx[MU > 0 ? MU : 1][CPL];
             if constexpr (NORB > 1) {
 #pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    if (p / NORB == p % NORB) continue;
+                for (int k = 0; k < MU; ++k)
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
-                        const int uu = mu[p][c] >= 0 ? (mu[p][c] & 0xffff) : iu + c;
-                        if (msf[p] >= 0) gs[p][c] = X[(size_t)(msf[p] & 0xffff) * P.ldx + uu];
-                        if (mpe[p] >= 0) gp[p][c] = X[(size_t)(mpe[p] & 0xffff) * P.ldx + uu];
+                        const int32_t u = ua[k][c];
+                        const int pu = u >> 24;  // -1 for an empty slot
+                        int32_t d = -1;
+#pragma unroll
+                        for (int p = 0; p < NP; ++p)
+                            if (p / NORB != p % NORB) d = pu == p ? dh[p] : d;
+                        dsel[k][c] = d;
+                        if (d >= 0) gx[k][c] = X[(size_t)(d & 0xffff) * P.ldx + (u & 0xffff)];
                     }
-                }
             }
             // ---- down-string moves in move order
 #pragma unroll
@@ -289,17 +318,15 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
             //      element J * sign_up * sign_dn (-J of ed_operators.F90:215,268 times the operator signs)
             if constexpr (NORB > 1) {
 #pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    if (p / NORB == p % NORB) continue;
+                for (int k = 0; k < MU; ++k)
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
-                        if (mu[p][c] >= 0) {
-                            const double cu = (mu[p][c] >> 16) ? -J : J;
-                            if (msf[p] >= 0) acc.v[c] = fma((msf[p] >> 16) ? -cu : cu, gs[p][c], acc.v[c]);
-                            if (mpe[p] >= 0) acc.v[c] = fma((mpe[p] >> 16) ? -cu : cu, gp[p][c], acc.v[c]);
+                        const int32_t d = dsel[k][c];
+                        if (d >= 0) {
+                            const double cu = ((ua[k][c] >> 16) & 1) ? -J : J;
+                            acc.v[c] = fma((d >> 16) ? -cu : cu, gx[k][c], acc.v[c]);
                         }
                     }
-                }
             }
             if (P.yscale) {
 #pragma unroll
