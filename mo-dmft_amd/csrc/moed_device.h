@@ -10,10 +10,55 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// wave_sum's value in lane 0 (the other lanes hold partial sums), with the same additions in the
+// same tree, built from VALU cross-lane moves instead of LDS permutes: lanes 0-31 take lane + 32
+// (v_permlane32_swap), lanes 0-15 lane + 16 (v_permlane16_swap), then DPP row shifts by 8, 4, 2, 1
+__device__ __forceinline__ double dbl_dpp_row_shl(double v, int n) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    switch (n) {  // DPP row_shl:n (dpp_ctrl 0x100 + n): lane i takes lane i + n of its 16-lane row
+        case 8:
+            lo = __builtin_amdgcn_update_dpp(lo, lo, 0x108, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(hi, hi, 0x108, 0xf, 0xf, false);
+            break;
+        case 4:
+            lo = __builtin_amdgcn_update_dpp(lo, lo, 0x104, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(hi, hi, 0x104, 0xf, 0xf, false);
+            break;
+        case 2:
+            lo = __builtin_amdgcn_update_dpp(lo, lo, 0x102, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(hi, hi, 0x102, 0xf, 0xf, false);
+            break;
+        default:
+            lo = __builtin_amdgcn_update_dpp(lo, lo, 0x101, 0xf, 0xf, false);
+            hi = __builtin_amdgcn_update_dpp(hi, hi, 0x101, 0xf, 0xf, false);
+            break;
+    }
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum0(double v) {
+    {
+        const int lo = __double2loint(v), hi = __double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        v += __hiloint2double(b[1], a[1]);
+    }
+    {
+        const int lo = __double2loint(v), hi = __double2hiint(v);
+        const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        v += __hiloint2double(b[1], a[1]);
+    }
+    v += dbl_dpp_row_shl(v, 8);
+    v += dbl_dpp_row_shl(v, 4);
+    v += dbl_dpp_row_shl(v, 2);
+    v += dbl_dpp_row_shl(v, 1);
+    return v;
+}
+
 // block sum in a fixed order (wave tree, then waves in index order); result valid in thread 0
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
-    v = wave_sum(v);
+    v = wave_sum0(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();

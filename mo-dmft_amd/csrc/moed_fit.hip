@@ -444,7 +444,7 @@ __device__ __forceinline__ double eval_func_tree_cached(const FitProb& qa) {
             s = fma(fma(re, re, im * im), rw, s);
         }
     }
-    s = wave_sum(s);
+    s = wave_sum0(s);
     if ((tid & 63) == 0) S.fred[0][tid >> 6] = s;
     __syncthreads();
     double tot = 0.0;
@@ -555,7 +555,7 @@ __device__ __forceinline__ void form_point(const FitView& q, double xv) {
 // the waves' partials of one evaluation, summed in block_sum's order by every thread
 __device__ __forceinline__ double point_total(const FitView& q, double s, int buf) {
 #pragma clang fp contract(off)
-    s = wave_sum(s);
+    s = wave_sum0(s);
     if ((threadIdx.x & 63) == 0) S.fred[buf][threadIdx.x >> 6] = s;
     __syncthreads();
     double tot = 0.0;
@@ -624,8 +624,8 @@ __device__ __forceinline__ void grad_group(const FitView& q, int o, const FreqGr
             pde = fma(v2 * fma(ure, d.re, -(uim * d.im)), rw2, pde);
             pdv = fma(2.0 * v * ure, rw2, pdv);
         }
-        pde = wave_sum(pde);
-        pdv = wave_sum(pdv);
+        pde = wave_sum0(pde);
+        pdv = wave_sum0(pdv);
         if (lane == 0) {
             S.gred[wid][k] = pde;
             S.gred[wid][kFitMaxBath + k] = pdv;
@@ -634,7 +634,7 @@ __device__ __forceinline__ void grad_group(const FitView& q, int o, const FreqGr
     double pdef = 0.0;
 #pragma unroll
     for (int j = 0; j < JN; ++j) pdef = fma(r.re[j], 2.0 * g.rw[j], pdef);
-    pdef = wave_sum(pdef);
+    pdef = wave_sum0(pdef);
     if (lane == 0) S.gred[wid][2 * kFitMaxBath] = pdef;
 }
 
