@@ -99,6 +99,8 @@ struct FitSmem {
     } wp[kFitThreads / 64];
     double fred[2][kFitThreads / 64];  // tree-order FUNC: the waves' partial sums (two buffers)
     int32_t tc_mode;                   // 1: u.tc, 2: u.tw holds the frequencies of this launch
+    int32_t cmd, cbuf;                 // the minimiser wave's posted command and its arguments
+    double cxv;
 };
 
 // the fit's LDS, shared by every device function of this file (one workgroup = one fit)
@@ -461,23 +463,21 @@ __device__ __forceinline__ double eval_func(const FitProb& q) {
     return uniform_i32(q.tree) ? eval_func_tree(q) : eval_func_ref(q);
 }
 
-// The couplings that can be nonzero on the line pcom + t xicom (wave 0, in k order, as
-// nonzero_couplings orders them); ends with a barrier.  A coupling on the list that is exactly zero
+// The couplings that can be nonzero on the line pcom + t xicom (in k order, as nonzero_couplings
+// orders them).  A coupling on the list that is exactly zero
 // at some t adds fma(0, d, r) = r to each sum (r starts at +0.0 and d is finite), so evaluations
 // over this list equal evaluations over the point's own nonzero list bit for bit.
-__device__ __forceinline__ void line_couplings(const FitView& q) {
+// Run by wave 0 alone (the minimiser wave); the workers read the lists after the next command.
+__device__ __forceinline__ void line_couplings_wave0(const FitView& q) {
     const int tid = threadIdx.x;
-    if (tid < 64) {
-        const int o = tid >> 4, k = tid & 15;
-        const bool in = o < q.norb && k < q.nbath;
-        const int j = in ? q.nbath + o * q.nbath + k : 0;
-        const bool nz = in && (S.pcom[j] != 0.0 || S.xicom[j] != 0.0);
-        const uint64_t mask = __ballot(nz);
-        const uint64_t seg = mask & (0xffffull << (16 * o));
-        if (nz) S.lk[o][__popcll(seg & ((1ull << tid) - 1ull))] = k;
-        if (k == 0 && o < q.norb) S.lnn[o] = __popcll(seg);
-    }
-    __syncthreads();
+    const int o = tid >> 4, k = tid & 15;
+    const bool in = o < q.norb && k < q.nbath;
+    const int j = in ? q.nbath + o * q.nbath + k : 0;
+    const bool nz = in && (S.pcom[j] != 0.0 || S.xicom[j] != 0.0);
+    const uint64_t mask = __ballot(nz);
+    const uint64_t seg = mask & (0xffffull << (16 * o));
+    if (nz) S.lk[o][__popcll(seg & ((1ull << tid) - 1ull))] = k;
+    if (k == 0 && o < q.norb) S.lnn[o] = __popcll(seg);
 }
 
 // mode 2: the terms of thread t's JN frequency slots, func_group's operations and order
@@ -715,31 +715,78 @@ __device__ __forceinline__ void eval_grad(const FitProb& q, double* out) {
         eval_grad_ref(q, out);
 }
 
-// f1dim (frprmn.f90:212-234)
+// ---- the minimiser wave and its workers
+//
+// Wave 0 runs frprmn / linmin / mnbrak / brent alone; the other waves only serve its block-wide
+// operations (a FUNC evaluation, a gradient).  For each one wave 0 writes the command (and its
+// arguments) to LDS and meets the workers at one barrier, after which every wave runs run_op with
+// the same command, so all waves pass the same barriers inside it.  A worker reads the command
+// before the first barrier inside the operation and wave 0 posts the next one only after that
+// barrier.  kCmdStop ends the workers' loop; the kernel posts it on every path out of the
+// minimiser.  The line-search logic, once repeated by all 16 waves (issue slots on every SIMD),
+// now runs in one.
+enum { kCmdStop = 0, kCmdFunc = 1, kCmdFuncListed = 2, kCmdFuncWide = 3, kCmdGrad = 4 };
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one block-wide operation; every wave of the block calls it with the same command
+__device__ __forceinline__ double run_op(const FitProb& q, int cmd) {
+    switch (cmd) {
+        case kCmdFuncListed:  // FUNC at xt with the coupling lists wave 0 built (cached frequencies)
+            return eval_func_tree_cached(q);
+        case kCmdFuncWide:  // FUNC at pcom + cxv xicom, frequencies cached kGrp per thread
+            return f1dim_tree_wide(q, S.cxv, S.cbuf);
+        case kCmdFunc:  // FUNC at xt
+            return eval_func(q);
+        default:  // kCmdGrad: dfunc at xt into xi
+            eval_grad(q, S.xi);
+            return 0.0;
+    }
+}
+
+// wave 0: post a command, meet the workers, run it
+__device__ __forceinline__ double command(const FitProb& q, int cmd) {
+    if (threadIdx.x == 0) S.cmd = cmd;
+    __syncthreads();
+    return cmd == kCmdStop ? 0.0 : run_op(q, cmd);
+}
+
+// waves 1..: serve commands until kCmdStop
+__device__ void serve(const FitProb& q) {
+    for (;;) {
+        __syncthreads();
+        const int cmd = uniform_i32(S.cmd);
+        if (cmd == kCmdStop) return;
+        (void)run_op(q, cmd);
+    }
+}
+
+// f1dim (frprmn.f90:212-234), run by wave 0: the point, then one command
 __device__ __forceinline__ double f1dim(const FitProb& q, double xv, int& nfev) {
 #pragma clang fp contract(off)
     ++nfev;
-    // tree order with cached frequencies: each wave forms the point itself (a non-finite xv takes
-    // the path below, where every coupling is formed from xt)
     const int mode = uniform_i32(S.tc_mode);
-    if (mode == 1) {  // wave 0 writes the point and builds its coupling lists, one barrier
-        if (threadIdx.x < 64) {
-            const int j = threadIdx.x;
-            if (j < q.nx) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            nonzero_couplings_wave0(view(q));
-        }
-        __syncthreads();
-        return eval_func_tree_cached(q);
-    }
+    const int lane = threadIdx.x;
     // up to kGrp frequencies per thread: each wave forms the point itself (a non-finite xv takes
     // the path below, where every coupling is formed from xt)
-    if (mode == 2 && isfinite(xv)) return f1dim_tree_wide(q, xv, nfev & 1);
-    for (int j = threadIdx.x; j < q.nx; j += kFitThreads) S.xt[j] = S.pcom[j] + xv * S.xicom[j];
-    __syncthreads();
-    return eval_func(q);
+    if (mode == 2 && isfinite(xv)) {
+        if (lane == 0) {
+            S.cxv = xv;
+            S.cbuf = nfev & 1;
+        }
+        return command(q, kCmdFuncWide);
+    }
+    if (lane < q.nx) S.xt[lane] = S.pcom[lane] + xv * S.xicom[lane];
+    wave_sync();
+    if (mode == 1) {  // one frequency per thread: wave 0 builds the point's coupling lists
+        nonzero_couplings_wave0(view(q));
+        return command(q, kCmdFuncListed);
+    }
+    return command(q, kCmdFunc);
 }
 
 __device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
@@ -889,70 +936,62 @@ __device__ bool brent(const FitProb q, double ax, double bx, double cx, double t
     return false;
 }
 
-// frprmn (frprmn.f90:1-84) with linmin (:86-117) inlined, for fit blockIdx.x
-__global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
+// frprmn (frprmn.f90:1-84) with linmin (:86-117) inlined, run by wave 0 of fit blockIdx.x; every
+// path returns here, where the kernel stops the workers.  LDS written by wave 0 reaches the
+// workers through the barrier of its next command.
+__device__ __forceinline__ void minimise(const FitCgArgs& a, int its0, int nfev, double fp, double fret) {
 #pragma clang fp contract(off)
     const FitProb q = a.q;
-    const int f = blockIdx.x, tid = threadIdx.x, n = q.nx;
+    const int f = blockIdx.x, lane = threadIdx.x, n = q.nx;  // n <= kFitNx = 64: one lane per entry
     int32_t* ist = a.ist + (size_t)f * kStateI;
     double* st = a.st + (size_t)f * kStateD;
     double* px = a.x + (size_t)f * n;
-    const int phase = uniform_i32(ist[0]);
-    if (phase == kPhaseDone) return;
-    tree_cache_fill(q);
     const double EPS = 1.e-10, LTOL = (double)1.e-8f;  // frprmn.f90:12, :91
-    int its0 = uniform_i32(ist[1]), nfev = uniform_i32(ist[3]);
-    double fp = st[3 * kFitNx], fret = st[3 * kFitNx + 1];
-    for (int j = tid; j < n; j += kFitThreads) {
-        S.p[j] = px[j];
-        S.xi[j] = st[j];
-        S.g[j] = st[kFitNx + j];
-        S.h[j] = st[2 * kFitNx + j];
-        S.xt[j] = px[j];
-    }
-    __syncthreads();
-    if (phase == kPhaseStart) {
-        fp = eval_func(q);
+    if (uniform_i32(ist[0]) == kPhaseStart) {
+        fp = command(q, kCmdFunc);  // FUNC at xt = the start point
         ++nfev;
-        eval_grad(q, S.xi);
-        for (int j = tid; j < n; j += kFitThreads) {
-            S.g[j] = -S.xi[j];
-            S.h[j] = S.g[j];
-            S.xi[j] = S.h[j];
+        command(q, kCmdGrad);
+        if (lane < n) {
+            S.g[lane] = -S.xi[lane];
+            S.h[lane] = S.g[lane];
+            S.xi[lane] = S.h[lane];
         }
-        __syncthreads();
+        wave_sync();
         its0 = 0;
     }
     int rc = 1, iter = its0;
     for (int its = its0 + 1; its <= kItmax; ++its) {
         iter = its;
         // ---- linmin
-        for (int j = tid; j < n; j += kFitThreads) {
-            S.pcom[j] = S.p[j];
-            S.xicom[j] = S.xi[j];
+        if (lane < n) {
+            S.pcom[lane] = S.p[lane];
+            S.xicom[lane] = S.xi[lane];
         }
-        __syncthreads();
-        if (uniform_i32(S.tc_mode) == 2) line_couplings(view(q));
+        wave_sync();
+        if (uniform_i32(S.tc_mode) == 2) {
+            line_couplings_wave0(view(q));
+            wave_sync();
+        }
         double ax = 0., xx = 1., bx, fa, fx, fb, xmin = 0.0, fr = 0.0;
         if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
             rc = MOED_ECONV;  // 'brent exceed maximum iterations'
             break;
         }
         fret = fr;
-        for (int j = tid; j < n; j += kFitThreads) {
-            S.xi[j] = xmin * S.xi[j];
-            S.p[j] = S.p[j] + S.xi[j];
+        if (lane < n) {
+            S.xi[lane] = xmin * S.xi[lane];
+            S.p[lane] = S.p[lane] + S.xi[lane];
         }
-        __syncthreads();
+        wave_sync();
         // ---- end linmin
         if (2.0 * fabs(fret - fp) <= a.ftol * (fabs(fret) + fabs(fp) + EPS)) {
             rc = MOED_OK;
             break;
         }
         fp = fret;
-        for (int j = tid; j < n; j += kFitThreads) S.xt[j] = S.p[j];
-        __syncthreads();
-        eval_grad(q, S.xi);
+        if (lane < n) S.xt[lane] = S.p[lane];
+        wave_sync();
+        command(q, kCmdGrad);
         double gg = 0.0, dgg = 0.0;
         for (int j = 0; j < n; ++j) {
             gg = gg + S.g[j] * S.g[j];
@@ -963,21 +1002,21 @@ __global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
             break;
         }
         const double gam = dgg / gg;
-        __syncthreads();
-        for (int j = tid; j < n; j += kFitThreads) {
-            S.g[j] = -S.xi[j];
-            S.h[j] = S.g[j] + gam * S.h[j];
-            S.xi[j] = S.h[j];
+        wave_sync();
+        if (lane < n) {
+            S.g[lane] = -S.xi[lane];
+            S.h[lane] = S.g[lane] + gam * S.h[lane];
+            S.xi[lane] = S.h[lane];
         }
-        __syncthreads();
+        wave_sync();
         if (its - its0 >= a.budget && its < kItmax) {  // suspend at the iteration boundary
-            for (int j = tid; j < n; j += kFitThreads) {
-                px[j] = S.p[j];
-                st[j] = S.xi[j];
-                st[kFitNx + j] = S.g[j];
-                st[2 * kFitNx + j] = S.h[j];
+            if (lane < n) {
+                px[lane] = S.p[lane];
+                st[lane] = S.xi[lane];
+                st[kFitNx + lane] = S.g[lane];
+                st[2 * kFitNx + lane] = S.h[lane];
             }
-            if (tid == 0) {
+            if (lane == 0) {
                 st[3 * kFitNx] = fp;
                 st[3 * kFitNx + 1] = fret;
                 ist[0] = kPhaseRun;
@@ -989,14 +1028,40 @@ __global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
         }
     }
     if (rc == 1) rc = MOED_ECONV;  // 'frprmn maximum iterations exceeded!'
-    for (int j = tid; j < n; j += kFitThreads) px[j] = S.p[j];
-    if (tid == 0) {
+    if (lane < n) px[lane] = S.p[lane];
+    if (lane == 0) {
         st[3 * kFitNx + 1] = fret;
         ist[0] = kPhaseDone;
         ist[1] = iter;
         ist[2] = rc;
         ist[3] = nfev;
     }
+}
+
+__global__ void __launch_bounds__(kFitThreads) fit_cg_kernel(FitCgArgs a) {
+    const FitProb q = a.q;
+    const int f = blockIdx.x, tid = threadIdx.x, n = q.nx;
+    const int32_t* ist = a.ist + (size_t)f * kStateI;
+    const double* st = a.st + (size_t)f * kStateD;
+    const double* px = a.x + (size_t)f * n;
+    if (uniform_i32(ist[0]) == kPhaseDone) return;
+    tree_cache_fill(q);
+    const int its0 = uniform_i32(ist[1]), nfev = uniform_i32(ist[3]);
+    const double fp = st[3 * kFitNx], fret = st[3 * kFitNx + 1];
+    for (int j = tid; j < n; j += kFitThreads) {
+        S.p[j] = px[j];
+        S.xi[j] = st[j];
+        S.g[j] = st[kFitNx + j];
+        S.h[j] = st[2 * kFitNx + j];
+        S.xt[j] = px[j];
+    }
+    __syncthreads();
+    if (uniform_i32(tid >> 6) != 0) {  // waves 1..: the workers
+        serve(q);
+        return;
+    }
+    minimise(a, its0, nfev, fp, fret);
+    (void)command(q, kCmdStop);
 }
 
 // FUNC and/or dfunc at x[blockIdx.x]
