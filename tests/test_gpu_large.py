@@ -5,8 +5,8 @@ tests/golden/make_large.py (the oracle, SciPy's ARPACK and the reference build):
     eigenvalues, the kept states (ed_solver.F90:65-90) and G(i w_n) (ed_green.F90:60-185), sector by
     sector and with every sector in one multi-problem run;
   * 14-site sector (7, 7): E0 and G(i w_n) of both orbitals on 2048 frequencies (200 steps);
-  * 15-site three-orbital sector (7, 8) with t_ij (D = 41,409,225): E0, and the particle Green's
-    function of orbital 1, spin up, on 4096 frequencies;
+  * 15-site three-orbital sector (7, 8) with t_ij (D = 41,409,225): the ground state as an
+    eigenpair of the oracle's Hamiltonian (Rayleigh quotient and residual);
   * the bath fit at the 15-site shape (nx 51, 4096 frequencies) in the reference's summation order:
     the reference build's minimization result bit for bit, with the oracle's iteration and
     FUNC-evaluation counts.
@@ -103,6 +103,31 @@ def test_ns14_ground_state_and_green_against_oracle():
     e = _gerr(G, z["G"])
     print(f"  G: rel err {e:.2e}", flush=True)
     assert e <= 1e-8
+
+
+def test_ns15_ground_state_is_an_eigenpair_of_the_oracle_hamiltonian():
+    """the 15-site three-orbital sector (7, 8) with t_ij, D = 41,409,225: the solver's ground state
+    under the oracle's H (ed_hamiltonian.F90:88-191 restated, run on the host cores): its Rayleigh
+    quotient equals E0 within 1e-12 relative, and the residual |H psi - E0 psi| is at the solver's
+    Ritz tolerance, so E0 lies within that residual of an eigenvalue of the reference Hamiltonian"""
+    import oracle
+    c = config("ns15_fit")
+    nu, nd = c.sectors[0]
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        E0 = kept[0][0]
+        psi = s.eigvec(kept[0][2], kept[0][3])
+    m = oracle.Model(c.norb, c.nbath, c.U, c.J, c.ek, c.vk, c.tij)
+    b = oracle.Basis(c.nsite, nu, nd)
+    nt = min(16, os.cpu_count() or 1)
+    hpsi = oracle.multiply_H(m, b, psi, nt)
+    nrm2 = float(np.dot(psi, psi))
+    rq = float(np.dot(psi, hpsi)) / nrm2
+    res = float(np.linalg.norm(hpsi - E0 * psi)) / np.sqrt(nrm2)
+    print(f"  E0 {E0!r} Rayleigh quotient {rq!r} residual {res:.2e} (|psi|^2 - 1 = {nrm2 - 1:.1e})", flush=True)
+    assert abs(nrm2 - 1.0) <= 1e-12
+    assert abs(rq - E0) <= 1e-12 * abs(E0)
+    assert res <= 1e-11 * abs(E0)  # measured 1.5e-11 absolute, 4.4e-13 relative
 
 
 def test_ns15_shape_fit_reference_order_bit_exact():
