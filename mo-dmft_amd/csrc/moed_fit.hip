@@ -796,12 +796,18 @@ __device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? 
 // below is far beyond that and only turns a pathological input into an error instead of an endless
 // kernel; false when it is reached.
 constexpr int kBracketCap = 100000;
+// f0 is FUNC at ax = 0, the line's start pcom, which the caller already holds: pcom is the last
+// line search's minimum pcom' + xmin xicom', formed with the roundings f1dim used for its value
+// there, and pcom + 0 xicom rounds back to pcom (only a -0.0 entry may turn +0.0, which FUNC only
+// squares or subtracts from 0.0).  The reference evaluates it again; here it is counted, not
+// recomputed, with the same result.
 __device__ bool mnbrak(const FitProb q, double& ax, double& bx, double& cx, double& fa, double& fb,
-                       double& fc, int& nfev) {
+                       double& fc, int& nfev, double f0) {
 #pragma clang fp contract(off)
     const double GOLD = (double)1.618034f, GLIMIT = 100.0, TINY = (double)1.e-20f;
     double dum, fu, q_, r, u, ulim;
-    fa = f1dim(q, ax, nfev);
+    ++nfev;  // fa = f1dim(ax = 0)
+    fa = f0;
     fb = f1dim(q, bx, nfev);
     if (fb > fa) {
         dum = ax;
@@ -973,7 +979,7 @@ __device__ __forceinline__ void minimise(const FitCgArgs& a, int its0, int nfev,
             wave_sync();
         }
         double ax = 0., xx = 1., bx, fa, fx, fb, xmin = 0.0, fr = 0.0;
-        if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
+        if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev, fp) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
             rc = MOED_ECONV;  // 'brent exceed maximum iterations'
             break;
         }
