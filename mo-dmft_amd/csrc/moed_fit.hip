@@ -867,9 +867,11 @@ __device__ bool mnbrak(const FitProb q, double& ax, double& bx, double& cx, doub
     return true;
 }
 
-// brent (frprmn.f90:119-210); false when its iteration limit is reached
+// brent (frprmn.f90:119-210); false when its iteration limit is reached.  fbx is FUNC at bx, which
+// mnbrak returned with it: mnbrak moves every abscissa together with its value, so fbx is the value
+// f1dim gave at this exact bx.  The reference evaluates it again; here it is counted, not recomputed.
 __device__ bool brent(const FitProb q, double ax, double bx, double cx, double tol, double& xmin,
-                      double& fret, int& nfev) {
+                      double& fret, int& nfev, double fbx) {
 #pragma clang fp contract(off)
     const double CGOLD = (double).3819660f, ZEPS = (double)1.0e-10f;
     double a = fmin(ax, cx), b = fmax(ax, cx);
@@ -877,7 +879,8 @@ __device__ bool brent(const FitProb q, double ax, double bx, double cx, double t
     v = bx;
     w = v;
     x = v;
-    fx = f1dim(q, x, nfev);
+    ++nfev;  // fx = f1dim(x = bx)
+    fx = fbx;
     fv = fx;
     fw = fx;
     for (int iter = 1; iter <= kBrentItmax; ++iter) {
@@ -979,7 +982,7 @@ __device__ __forceinline__ void minimise(const FitCgArgs& a, int its0, int nfev,
             wave_sync();
         }
         double ax = 0., xx = 1., bx, fa, fx, fb, xmin = 0.0, fr = 0.0;
-        if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev, fp) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev)) {
+        if (!mnbrak(q, ax, xx, bx, fa, fx, fb, nfev, fp) || !brent(q, ax, xx, bx, LTOL, xmin, fr, nfev, fx)) {
             rc = MOED_ECONV;  // 'brent exceed maximum iterations'
             break;
         }
