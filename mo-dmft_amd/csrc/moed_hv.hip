@@ -151,6 +151,148 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     }
 }
 
+// Resident row pass for sectors whose row and its negation fit twice in LDS (nU <= 4094): one
+// 1024-thread block per compute unit works through rows blockIdx.x, blockIdx.x + gridDim.x, ...
+// Each thread keeps the byte-offset records of its columns (iu = tid + 1024 c, c < 4) in
+// registers, loaded once, instead of reading them from L2 for every row (at 14 sites the per-row
+// kernel reads 425 MB of records per product).  The row is staged in two LDS buffers of 2 nU + 2
+// doubles (layout as in hv_rows_kernel).  Step k writes the registers holding row k + 1 to the
+// buffer step k - 1 read, issues the loads of row k + 2 into them, and computes row k from the
+// other buffer, so a row's loads are in flight for a whole step; one barrier per step.  Every
+// element is computed by hv_rows_kernel's operations in its order (bit-identical).
+constexpr int kResRowsMaxU = 4094;
+constexpr int kResRowsBuf = 2 * kResRowsMaxU + 2;  // doubles per buffer: 65,520 bytes, so the
+                                                   // second buffer is an immediate LDS offset
+constexpr int kResRowsCpt = 4;                     // columns per thread: ceil(4094 / 1024)
+
+// a thread's share of one row: 16-byte pairs p = tid + 1024 j (PAIRS) or single doubles
+template <bool PAIRS>
+struct RowRegs {
+    double v[4];
+};
+
+template <bool PAIRS>
+__device__ __forceinline__ void persist_load(RowRegs<PAIRS>& r, const double* __restrict__ xr, int nU, int tid) {
+#pragma unroll
+    for (int j = 0; j < (PAIRS ? 2 : 4); ++j) {
+        if constexpr (PAIRS) {
+            const int p = tid + 1024 * j;
+            if (p < nU / 2) {
+                const double2 t = reinterpret_cast<const double2*>(xr)[p];
+                r.v[2 * j] = t.x;
+                r.v[2 * j + 1] = t.y;
+            }
+        } else {
+            const int k = tid + 1024 * j;
+            if (k < nU) r.v[j] = xr[k];
+        }
+    }
+}
+
+template <bool PAIRS>
+__device__ __forceinline__ void persist_store(const RowRegs<PAIRS>& r, double* xs, int nU, int tid) {
+    const int neg = nU + 1;
+#pragma unroll
+    for (int j = 0; j < (PAIRS ? 2 : 4); ++j) {
+        if constexpr (PAIRS) {
+            const int p = tid + 1024 * j;
+            if (p < nU / 2) {
+                reinterpret_cast<double2*>(xs)[p] = make_double2(r.v[2 * j], r.v[2 * j + 1]);
+                xs[neg + 2 * p] = -r.v[2 * j];
+                xs[neg + 2 * p + 1] = -r.v[2 * j + 1];
+            }
+        } else {
+            const int k = tid + 1024 * j;
+            if (k < nU) {
+                xs[k] = r.v[j];
+                xs[neg + k] = -r.v[j];
+            }
+        }
+    }
+}
+
+// one step: stage row idn (registers -> buffer 1 - B), load row idl into registers, compute row id
+// from buffer B (idn, idl < 0: none)
+template <int NORB, int NW, int NDP, int B, bool PAIRS>
+__device__ __forceinline__ void persist_step(const HvArgs& P, double* xb, const uint32_t (&w)[kResRowsCpt][NW],
+                                             RowRegs<PAIRS>& rr, int id, int idn, int idl) {
+    constexpr int KMAX = 2 * NW - 5;
+    const int nU = P.up.count, tid = threadIdx.x;
+    if (idn >= 0) persist_store<PAIRS>(rr, xb + (1 - B) * kResRowsBuf, nU, tid);
+    if (idl >= 0) persist_load<PAIRS>(rr, P.x + (size_t)idl * P.ldx, nU, tid);
+
+    const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
+    const uint32_t sd = scalar_u16(P.dn.str, id);
+    double ekr[NDP];
+    {
+        const cptr<double> ekc = as_const(P.ek);
+        uint32_t m = sd;
+#pragma unroll
+        for (int q = 0; q < NDP; ++q) {
+            const double e = ekc[m ? __builtin_ctz(m) : 0];
+            ekr[q] = m ? e : 0.0;
+            m &= m - 1;
+        }
+    }
+    const RowDiag<NORB> rd = row_diag<NORB>(sd, U, UmJ3, Um2J);
+    const cptr<double> slotc = as_const(P.slotc);
+    const char* base = reinterpret_cast<const char*>(xb + B * kResRowsBuf);
+    double* __restrict__ yr = P.y + (size_t)id * P.ldx;
+#pragma unroll
+    for (int c = 0; c < kResRowsCpt; ++c) {
+        const int iu = tid + 1024 * c;
+        if (iu < nU) {
+            double S = __hiloint2double((int)w[c][1], (int)w[c][0]);
+#pragma unroll
+            for (int q = 0; q < NDP; ++q) S += ekr[q];
+            double acc = __dmul_rn(diag_interaction<NORB>(S, w[c][2] & 0xffffu, rd, UmJ3),
+                                   *reinterpret_cast<const double*>(base + 8 * iu));
+#pragma unroll
+            for (int q = 0; q < KMAX; ++q) {
+                const uint32_t e = rec_u16(w[c], 5 + q);
+                acc = fma(slotc[q], *reinterpret_cast<const double*>(base + e), acc);
+            }
+            yr[iu] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+template <int NORB, int NW, int NDP, bool PAIRS>
+__global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
+    __shared__ __attribute__((aligned(16))) double xb[2 * kResRowsBuf];
+    if (P.stop && *P.stop) return;
+    const int nU = P.up.count, nD = P.dn.count, tid = threadIdx.x;
+    const int G = gridDim.x, r0 = blockIdx.x;
+    const int K = (nD - r0 + G - 1) / G;  // rows of this block: r0 + G k, k < K (K >= 1)
+    const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(P.up.rec8);
+    const int rstride = 4 * P.up.nq;
+    uint32_t w[kResRowsCpt][NW];
+#pragma unroll
+    for (int c = 0; c < kResRowsCpt; ++c) {
+        const int iu = min(tid + 1024 * c, nU - 1);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) w[c][q] = recw[(size_t)iu * rstride + q];
+    }
+    // prologue: row 0 of the block into buffer 0, row 1 into registers, the zero entries of both buffers
+    RowRegs<PAIRS> rr;
+    persist_load<PAIRS>(rr, P.x + (size_t)r0 * P.ldx, nU, tid);
+    persist_store<PAIRS>(rr, xb, nU, tid);
+    if (K > 1) persist_load<PAIRS>(rr, P.x + (size_t)(r0 + G) * P.ldx, nU, tid);
+    if (tid < 2) {
+        xb[tid * kResRowsBuf + nU] = 0.0;
+        xb[tid * kResRowsBuf + 2 * nU + 1] = 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < K; k += 2) {
+        persist_step<NORB, NW, NDP, 0, PAIRS>(P, xb, w, rr, r0 + G * k, k + 1 < K ? r0 + G * (k + 1) : -1,
+                                              k + 2 < K ? r0 + G * (k + 2) : -1);
+        if (k + 1 >= K) break;
+        persist_step<NORB, NW, NDP, 1, PAIRS>(P, xb, w, rr, r0 + G * (k + 1), k + 2 < K ? r0 + G * (k + 2) : -1,
+                                              k + 3 < K ? r0 + G * (k + 3) : -1);
+    }
+}
+
 // ------------------------------------------------------------------ strip pass
 
 // CPL consecutive columns of one row as one load (CPL = 2 needs an even row stride: 16-byte aligned)
@@ -368,9 +510,11 @@ int record_dwords(int nslot) {
 // ncu: compute units of the device
 HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
     HvPlan p{};
+    // the default shape (threads_a 0) takes the resident row pass where its records fill the
+    // threads (2048 <= nU <= kResRowsMaxU) and every compute unit gets two rows or more
+    p.grid_p = (threads_a <= 0 && nU >= 2048 && nU <= kResRowsMaxU && ncu > 0 && nD >= 2 * ncu) ? ncu : 0;
     p.threads_a = threads_a > 0 ? threads_a : 1024;
     p.grid_a = nD;
-    (void)ncu;
     if (strip <= 0) strip = 128;
     p.cpl = (strip >= 128 && (ldx % 2) == 0 && (nU % 2 == 0 || nU >= 3)) ? 2 : 1;
     // rows per work item: 8 per wave with 128-column strips, 32 with 64-column strips (items of
@@ -386,6 +530,16 @@ HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
 
 template <int NORB, int NW, int NDP>
 static hipError_t rows_launch_nd(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
+    if constexpr (NW <= 12) {
+        if (pl.grid_p > 0 && a.up.rec8 && a.up.count <= kResRowsMaxU) {
+            const bool pairs = (a.up.count % 2) == 0 && (a.ldx % 2) == 0 && (reinterpret_cast<uintptr_t>(a.x) & 15) == 0;
+            if (pairs)
+                hipLaunchKernelGGL((hv_rows_persist_kernel<NORB, NW, NDP, true>), dim3(pl.grid_p), dim3(1024), 0, s, a);
+            else
+                hipLaunchKernelGGL((hv_rows_persist_kernel<NORB, NW, NDP, false>), dim3(pl.grid_p), dim3(1024), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     if (a.up.rec8 && a.up.count <= kFastMaxU) {
         hipLaunchKernelGGL((hv_rows_kernel<NORB, NW, NDP, true>), dim3(pl.grid_a), dim3(pl.threads_a), 0, s, a);
         return hipGetLastError();

@@ -75,6 +75,7 @@ constexpr int kStripThreads = 256;
 // launch geometry of the two passes for a sector with nU x nD elements
 struct HvPlan {
     int threads_a, grid_a;              // row pass: one block per row
+    int grid_p;                         // persistent row pass: blocks (0: not used)
     int cpl;                            // strip pass: columns per lane (strip width 64 * cpl)
     int rows_b, items_per_xcd, grid_b;  // grid_b = number of x.y partials
 };

@@ -4,8 +4,9 @@ the winners to ``tuning.json`` beside this file, which ``EDSolver.from_config`` 
 Tuned per workload (every candidate gives bit-identical results, so tuning changes only speed):
 
   * the sector H v (multiply_H, ed_hamiltonian.F90:88-191) of the workload's largest sector: row
-    pass threads per block (256, 512, 1024) x strip width in columns (64, 128), timed over HIP-graph
-    replays of the product on resident vectors;
+    pass shape (0: the default, resident blocks where the sector allows them; 256, 512, 1024: one
+    block of that size per row) x strip width in columns (64, 128), timed over HIP-graph replays
+    of the product on resident vectors;
   * for workloads with many sectors, the largest sector that the batched solve runs in its
     persistent work-queue launch (none, 2048, 8192, 32768, all), timed over whole solves.
 
@@ -25,7 +26,7 @@ from .model import config
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-HV_SHAPES = [(t, w) for t in (256, 512, 1024) for w in (64, 128)]
+HV_SHAPES = [(t, w) for t in (0, 256, 512, 1024) for w in (64, 128)]
 PERSIST = [0, 2048, 8192, 32768, 1 << 20]
 
 

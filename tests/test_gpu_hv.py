@@ -139,6 +139,28 @@ def test_hv_14site_properties_and_oracle(configs):
     assert np.max(np.abs(Hx - yo)) <= 1e-14 * np.abs(yo).max()
 
 
+@pytest.mark.parametrize("sector", [(7, 7), (6, 7)])
+def test_hv_14site_resident_row_pass_equals_per_row_blocks(configs, sector):
+    """the default row pass at 14 sites (resident blocks with the records in registers; nU = 3432
+    stages 16-byte pairs, nU = 3003 single doubles) against one block per row of 1024 and 512
+    threads: bit-identical; (6, 7) also against the oracle"""
+    c = configs["ns14_roofline"]
+    nu, nd = sector
+    with EDSolver.from_config(c, tuned=False) as s:
+        D = s.dim(nu, nd)
+        x = start_vector(nu * 7 + nd, D)
+        ys = []
+        for threads in (0, 1024, 512):
+            s.set_hv_shape(threads, 0)
+            ys.append(s.multiply_H(nu, nd, x))
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0])
+    if sector == (6, 7):
+        m = omodel(c.norb, c.nbath, c.U, c.J, c.ek, c.vk)
+        yo = oracle.multiply_H(m, oracle.Basis(c.nsite, nu, nd), x, min(16, os.cpu_count() or 1))
+        assert np.max(np.abs(ys[0] - yo)) <= 1e-14 * np.abs(yo).max()
+
+
 def test_hv_15site_tij_rows_and_properties(configs):
     """the largest sector: 15 sites, 3 orbitals with impurity hopping, (7, 8), D = 41,409,225 (odd
     nU = 6435): 400 rows against the oracle's rows of H (first and last rows, every row-block edge,
@@ -201,13 +223,15 @@ def test_hv_slot_counts_and_sector_shapes(case):
 
 
 @pytest.mark.parametrize("name, sector", [("ns15_fit", (2, 2)), ("ns15_fit", (8, 1)), ("ns15_fit", (1, 14)),
-                                          ("ns15_fit", (6, 5)), ("ns12_plain", (6, 6)), ("ns12_plain", (5, 6))])
+                                          ("ns15_fit", (6, 5)), ("ns12_plain", (6, 6)), ("ns12_plain", (5, 6)),
+                                          ("ns14_roofline", (7, 7))])
 def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
     """moed_hv_device with x and y starting at either 8-byte phase (odd nU at 15 sites, where every
-    C(15, k) is odd, and even nU at 12 sites, whose 16-byte row and strip loads need 16-byte aligned
-    vectors and otherwise fall back to 8-byte loads): y is bit-identical to the host-buffer product
-    and to the 64-column (8-byte load) strips in every case, and nothing is written outside y.  The
-    device buffers come from the HIP runtime the library uses."""
+    C(15, k) is odd, and even nU at 12 and 14 sites, whose 16-byte row and strip loads need 16-byte
+    aligned vectors and otherwise fall back to 8-byte loads; at 14 sites the row pass is the
+    resident one): y is bit-identical to the host-buffer product and to the 64-column (8-byte load)
+    strips in every case, and nothing is written outside y.  The device buffers come from the HIP
+    runtime the library uses."""
     import ctypes as ct
     hip = ct.CDLL("libamdhip64.so")
     hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
