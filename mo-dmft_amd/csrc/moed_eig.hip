@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(kEigThreads) hv_batch_kernel(HvBatchArgs A) {
 
 template <int NORB, int NW>
 static hipError_t hv_batch_t(const HvBatchArgs& a, int nitems, int max_nU, hipStream_t s) {
-    const size_t lds = (2 * (size_t)max_nU + 2) * sizeof(double);
+    const size_t lds = (size_t)row_len(max_nU) * sizeof(double);
     auto kern = hv_batch_kernel<NORB, NW>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -353,12 +353,12 @@ int build_table(moed_handle* h, int n, StrTable** out) {
             const bool ok = sl.imp ? (bq && !bp) : (bp != bq);
             if (!ok) continue;  // entry C: the zero
             const uint32_t tgt = s ^ (1u << sl.p) ^ (1u << sl.q);
-            rr[5 + k] = (uint16_t)(move_parity(s, sl.p, sl.q) ? C + 1 + t->rank[tgt] : t->rank[tgt]);
+            rr[5 + k] = (uint16_t)(move_parity(s, sl.p, sl.q) ? row_neg(C) + t->rank[tgt] : t->rank[tgt]);
         }
     }
     HIPCHK(t->d_rec.ensure(rec.size() * sizeof(uint16_t)));
     HIPCHK(hipMemcpy(t->d_rec.p, rec.data(), rec.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-    if (8 * (2 * (int64_t)C + 1) < 65536) {  // the same records with slot entries as LDS byte offsets
+    if (8 * (int64_t)row_len(C) <= 65536) {  // the same records with slot entries as LDS byte offsets
         for (int r = 0; r < C; ++r)  // every slot position, the padding slots' zero entries included
             for (size_t k = 5; k < ru16; ++k) rec[ru16 * r + k] = (uint16_t)(8 * rec[ru16 * r + k]);
         HIPCHK(t->d_rec8.ensure(rec.size() * sizeof(uint16_t)));

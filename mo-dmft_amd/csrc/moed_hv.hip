@@ -32,7 +32,7 @@ namespace moed {
 // ------------------------------------------------------------------ row pass
 
 // Row pass, one block per row (down string id).  The row is staged in LDS twice, as x and -x:
-//   entry k (0 <= k < nU): x[id][k];  entry nU: 0.0;  entry nU+1+k: -x[id][k]
+//   entry k (0 <= k < nU): x[id][k];  entry nU: 0.0;  entry row_neg(nU) + k: -x[id][k]
 // A move with sign -1 reads the negated copy, so every slot term is one fma with the slot's
 // coefficient, and slots that do not apply (and the padding slots past nslot, coefficient 0.0)
 // read the zero: the KMAX = 2 NW - 5 slot reads carry no branches and are all in flight
@@ -52,7 +52,7 @@ constexpr int kFastMaxU = 4095;
 template <int NORB, int NW, int NDP, bool BYTES>
 __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     constexpr int KMAX = 2 * NW - 5;
-    __shared__ double xstat[BYTES ? 2 * kFastMaxU + 2 : 1];
+    __shared__ __attribute__((aligned(16))) double xstat[BYTES ? row_len(kFastMaxU) : 1];
     extern __shared__ double xdyn[];
     double* const xs = BYTES ? xstat : xdyn;
 
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     const int NT = blockDim.x;
     const int nU = P.up.count;
     const int id = blockIdx.x;
-    const int neg = nU + 1;
+    const int neg = row_neg(nU);
     const int tid = threadIdx.x;
     const double* __restrict__ xr = P.x + (size_t)id * P.ldx;
     double* __restrict__ yr = P.y + (size_t)id * P.ldx;
@@ -86,8 +86,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
                 const int p = p0 + u * NT;
                 if (p < nU / 2) {
                     reinterpret_cast<double2*>(xs)[p] = v[u];
-                    xs[neg + 2 * p] = -v[u].x;
-                    xs[neg + 2 * p + 1] = -v[u].y;
+                    reinterpret_cast<double2*>(xs + neg)[p] = make_double2(-v[u].x, -v[u].y);
                 }
             }
         }
@@ -124,7 +123,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     const RowDiag<NORB> rd = row_diag<NORB>(sd, U, UmJ3, Um2J);
     if (tid == 0) {
         xs[nU] = 0.0;
-        xs[2 * nU + 1] = 0.0;
+        xs[neg + nU] = 0.0;
     }
     __syncthreads();
 
@@ -151,19 +150,20 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     }
 }
 
-// Resident row pass for sectors whose row and its negation fit twice in LDS (nU <= 4094): one
+// Resident row pass for sectors whose row and its negation fit twice in LDS (nU <= 4093): one
 // 1024-thread block per compute unit works through rows blockIdx.x, blockIdx.x + gridDim.x, ...
 // Each thread keeps the byte-offset records of its columns (iu = tid + 1024 c, c < 4) in
 // registers, loaded once, instead of reading them from L2 for every row (at 14 sites the per-row
-// kernel reads 425 MB of records per product).  The row is staged in two LDS buffers of 2 nU + 2
-// doubles (layout as in hv_rows_kernel).  Step k writes the registers holding row k + 1 to the
+// kernel reads 425 MB of records per product).  The row is staged in two LDS buffers of
+// row_len(nU) doubles (layout as in hv_rows_kernel).  Step k writes the registers holding row k + 1 to the
 // buffer step k - 1 read, issues the loads of row k + 2 into them, and computes row k from the
 // other buffer, so a row's loads are in flight for a whole step; one barrier per step.  Every
 // element is computed by hv_rows_kernel's operations in its order (bit-identical).
-constexpr int kResRowsMaxU = 4094;
-constexpr int kResRowsBuf = 2 * kResRowsMaxU + 2;  // doubles per buffer: 65,520 bytes, so the
-                                                   // second buffer is an immediate LDS offset
-constexpr int kResRowsCpt = 4;                     // columns per thread: ceil(4094 / 1024)
+constexpr int kResRowsMaxU = 4093;
+constexpr int kResRowsBuf = 8190;  // doubles per buffer: 65,520 bytes, so the second buffer is an
+                                   // immediate LDS offset, and 16-byte aligned
+constexpr int kResRowsCpt = 4;     // columns per thread: ceil(4093 / 1024)
+static_assert(row_len(kResRowsMaxU) <= kResRowsBuf, "a staged row fits its buffer");
 
 // a thread's share of one row: 16-byte pairs p = tid + 1024 j (PAIRS) or single doubles
 template <bool PAIRS>
@@ -191,15 +191,14 @@ __device__ __forceinline__ void persist_load(RowRegs<PAIRS>& r, const double* __
 
 template <bool PAIRS>
 __device__ __forceinline__ void persist_store(const RowRegs<PAIRS>& r, double* xs, int nU, int tid) {
-    const int neg = nU + 1;
+    const int neg = row_neg(nU);
 #pragma unroll
     for (int j = 0; j < (PAIRS ? 2 : 4); ++j) {
         if constexpr (PAIRS) {
             const int p = tid + 1024 * j;
             if (p < nU / 2) {
                 reinterpret_cast<double2*>(xs)[p] = make_double2(r.v[2 * j], r.v[2 * j + 1]);
-                xs[neg + 2 * p] = -r.v[2 * j];
-                xs[neg + 2 * p + 1] = -r.v[2 * j + 1];
+                reinterpret_cast<double2*>(xs + neg)[p] = make_double2(-r.v[2 * j], -r.v[2 * j + 1]);
             }
         } else {
             const int k = tid + 1024 * j;
@@ -281,7 +280,7 @@ __global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
     if (K > 1) persist_load<PAIRS>(rr, P.x + (size_t)(r0 + G) * P.ldx, nU, tid);
     if (tid < 2) {
         xb[tid * kResRowsBuf + nU] = 0.0;
-        xb[tid * kResRowsBuf + 2 * nU + 1] = 0.0;
+        xb[tid * kResRowsBuf + row_neg(nU) + nU] = 0.0;
     }
     __syncthreads();
     for (int k = 0; k < K; k += 2) {
@@ -544,7 +543,7 @@ static hipError_t rows_launch_nd(const HvArgs& a, const HvPlan& pl, hipStream_t 
         hipLaunchKernelGGL((hv_rows_kernel<NORB, NW, NDP, true>), dim3(pl.grid_a), dim3(pl.threads_a), 0, s, a);
         return hipGetLastError();
     }
-    const size_t lds = (2 * (size_t)a.up.count + 2) * sizeof(double);
+    const size_t lds = (size_t)row_len(a.up.count) * sizeof(double);
     auto kern = hv_rows_kernel<NORB, NW, NDP, false>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -10,11 +10,11 @@
 namespace moed {
 
 // Row id of a small sector, single pass: thread t of nthr stages the row and its negation in xs
-// (entry nU holds 0.0, entries nU + 1 + k hold -x[k]) and thread 0 the energies of the row's
+// (the layout of row_neg / row_len: 0.0 at nU, -x[k] at row_neg(nU) + k) and thread 0 the energies of the row's
 // occupied down sites in ekd.  The caller synchronises before hv_row_compute.
 __device__ __forceinline__ void hv_row_stage(const BatchProbDev& d, int id, const double* __restrict__ X, double* xs,
                                              double* ekd, const double* ek, int t, int nthr) {
-    const int nU = d.up.count, neg = nU + 1;
+    const int nU = d.up.count, neg = row_neg(nU);
     const double* __restrict__ xr = X + (size_t)id * nU;
     for (int k = t; k < nU; k += nthr) {
         const double v = xr[k];
@@ -23,7 +23,7 @@ __device__ __forceinline__ void hv_row_stage(const BatchProbDev& d, int id, cons
     }
     if (t == 0) {
         xs[nU] = 0.0;
-        xs[2 * nU + 1] = 0.0;
+        xs[neg + nU] = 0.0;
         const cptr<double> ekc = as_const(ek);
         uint32_t m = scalar_u16(d.dn.str, id);
         for (int q = 0; m; m &= m - 1, ++q) ekd[q] = ekc[__builtin_ctz(m)];

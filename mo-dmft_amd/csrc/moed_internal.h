@@ -32,9 +32,17 @@ constexpr int kXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 //                          ascending site order from 0.0, the first partial sum of multiply_H's
 //                          diagonal, ed_hamiltonian.F90:112-123) as a double; [4] the string;
 //                          [5 + k] for slot k < nslot: the staged-row entry the move reads, rank
-//                          (sign +), count + 1 + rank (sign -, the negated copy), or count (zero)
-//                          when the slot does not apply.  A slot is a fixed (orbital, bath) pair or
-//                          an ordered impurity pair with coefficient slotc[k].
+//                          (sign +), row_neg(count) + rank (sign -, the negated copy), or count
+//                          (zero) when the slot does not apply.  A slot is a fixed (orbital, bath)
+//                          pair or an ordered impurity pair with coefficient slotc[k].
+// A row staged for the row pass (LDS, row_len(nU) doubles): x[k] at entry k < nU, 0.0 at nU, -x[k]
+// at row_neg(nU) + k, 0.0 at row_neg(nU) + nU.  The negated copy starts at a multiple of 32
+// doubles (the period of the LDS banks), so a gather's bank is the same whichever copy its sign
+// selects: the up moves of neighbouring columns reach nearby ranks, and at 14 sites this cuts the
+// bank-conflict cycles of the slot gathers by a quarter against a copy at nU + 1.
+__host__ __device__ constexpr int row_neg(int nU) { return (nU + 32) & ~31; }
+__host__ __device__ constexpr int row_len(int nU) { return row_neg(nU) + nU + 1; }
+
 struct StrDev {
     const uint16_t* str;
     const int32_t* nhop;
@@ -42,7 +50,7 @@ struct StrDev {
     const double* mvc;
     const int32_t* imp;
     const uint4* rec;
-    const uint4* rec8;  // the records with slot entries as byte offsets (8 entry); nullptr unless 8 (2 count + 1) < 65536
+    const uint4* rec8;  // the records with slot entries as byte offsets (8 entry); nullptr unless 8 row_len(count) <= 65536
     int32_t count;
     int32_t mh;
     int32_t nslot;

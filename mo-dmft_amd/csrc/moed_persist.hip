@@ -80,13 +80,13 @@ struct PersistShared {
 
 template <int NORB, int NW>
 __global__ void __launch_bounds__(kPT) eig_persist_kernel(PersistArgs A) {
-    extern __shared__ double dyn[];  // kPQ staged rows of 2 max_nU + 2 doubles, then alpha, beta^2, beta
+    extern __shared__ double dyn[];  // kPQ staged rows of row_len(max_nU) doubles, then alpha, beta^2, beta
     __shared__ PersistShared S;
     __shared__ int s_next;
     const EigArgs& e = A.e;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int quarter = uniform_i32(tid >> 8), t256 = tid & 255;
-    const int rowlen = 2 * A.max_nU + 2;
+    const int rowlen = row_len(A.max_nU);
     double* xs = dyn + (size_t)quarter * rowlen;
     double* ta = dyn + (size_t)kPQ * rowlen;  // T_m for the convergence test
     double* tb2 = ta + e.ldh;
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(kPT) eig_persist_kernel(PersistArgs A) {
 
 template <int NORB, int NW>
 static hipError_t persist_t(const PersistArgs& a, int grid, hipStream_t s) {
-    const size_t lds = ((size_t)kPQ * (2 * a.max_nU + 2) + 3 * (size_t)a.e.ldh) * sizeof(double);
+    const size_t lds = ((size_t)kPQ * row_len(a.max_nU) + 3 * (size_t)a.e.ldh) * sizeof(double);
     auto kern = eig_persist_kernel<NORB, NW>;
     if (lds > 64 * 1024) {
         hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
