@@ -225,10 +225,10 @@ int moed_fill_start_device(moed_handle* h, uint64_t seed, int64_t n, double* x_d
 
 /* launch shape of the H v kernels (no counterpart in the reference): row pass threads per block
  * (0 = default: one resident 1024-thread block per compute unit that keeps its columns' records in
- * registers, where the sector has 2048 to 4094 up strings and at least two rows per compute unit,
+ * registers, where the sector has 2048 to 4093 up strings and at least two rows per compute unit,
  * else one 1024-thread block per row; 256, 512 or 1024 = one block of that size per row) and strip
- * pass width in columns (0 = default 128, 64 or 128; 64 where the up-string count is odd).
- * Results are bit-identical for every shape. */
+ * pass width in columns (0 = default: 128 where the up-string count is even, else 64; 64 or 128).
+ * Results are bit-identical for every shape.  Vectors passed to the H v need 8-byte alignment. */
 int moed_set_hv_shape(moed_handle* h, int32_t threads, int32_t strip);
 /* sectors of at most max_dim elements that moed_solve with MOED_F_BATCHED runs in its persistent
  * work-queue launch (no counterpart in the reference): the whole Lanczos iteration of each such

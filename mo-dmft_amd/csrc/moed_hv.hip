@@ -294,16 +294,19 @@ __global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
 
 // ------------------------------------------------------------------ strip pass
 
-// CPL consecutive columns of one row as one load (CPL = 2 needs an even row stride: 16-byte aligned)
+// CPL consecutive columns of one row as one load.  CPL = 2 is one 16-byte load, which gfx950
+// serves at any 8-byte alignment (the HSA runtime enables unaligned access), so rows of odd
+// stride take it too; the type below tells the compiler the address is only 8-byte aligned.
 template <int CPL>
 struct Cols {
     double v[CPL];
 };
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
 template <int CPL>
 __device__ __forceinline__ Cols<CPL> load_cols(const double* p) {
     Cols<CPL> c;
     if constexpr (CPL == 2) {
-        const double2 t = *reinterpret_cast<const double2*>(p);
+        const d2u t = *reinterpret_cast<const d2u*>(p);
         c.v[0] = t.x;
         c.v[1] = t.y;
     } else {
@@ -348,10 +351,12 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
         const int strip = item / bps, rb = item % bps;
         const int c00 = strip * W + lane * CPL;
         const bool ok = c00 < nU;
-        // with CPL = 2 and an odd nU (rows padded to an even stride) the last pair ends in the
-        // padding column: it is read (zero) but never stored; idle lanes read an aligned valid pair
-        const bool ok1 = CPL == 1 || c00 + 1 < nU;
-        const int iu = ok ? c00 : ((nU - CPL) & ~(CPL - 1));
+        // with CPL = 2 and an odd nU the lane of the last column takes the pair (nU - 2, nU - 1) and
+        // keeps only its second column (each column's terms do not depend on its place in the pair),
+        // so no load reaches past a row; idle lanes read a valid pair and keep nothing
+        const bool tail = CPL == 2 && ok && c00 + 1 >= nU;
+        const int iu = ok ? (tail ? nU - 2 : c00) : ((nU - CPL) & ~(CPL - 1));
+        const bool ok1 = CPL == 1 || iu + 1 < nU;
         const int r1 = min(nD, (rb + 1) * P.rows_b);
         // up half of the spin-flip / pair-exchange terms, compacted per column: pair p = (i, j) needs
         // up j occupied and up i empty (move j -> i), which holds for at most MU of the ordered pairs
@@ -474,16 +479,15 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 for (int c = 0; c < CPL; ++c) acc.v[c] *= inv;
             }
             if (ok) {
-                if (CPL == 2 && ok1) {  // one 16-byte store (two 8-byte ones write ~2 % more)
-                    typedef double d2v __attribute__((ext_vector_type(2)));
-                    const d2v v = {acc.v[0], acc.v[CPL - 1]};
-                    __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(Y + ro + iu));
+                if (CPL == 2 && !tail) {  // one 16-byte store (two 8-byte ones write ~2 % more)
+                    const d2u v = {acc.v[0], acc.v[CPL - 1]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<d2u*>(Y + ro + iu));
                 } else {
-                    __builtin_nontemporal_store(acc.v[0], Y + ro + iu);
+                    __builtin_nontemporal_store(acc.v[CPL - 1], Y + ro + iu + CPL - 1);
                 }
                 if (P.dot_part) {
-                    dsum = fma(xo.v[0], acc.v[0], dsum);
-                    if (CPL == 2 && ok1) dsum = fma(xo.v[CPL - 1], acc.v[CPL - 1], dsum);
+                    if (CPL == 1 || !tail) dsum = fma(xo.v[0], acc.v[0], dsum);
+                    if (CPL == 2) dsum = fma(xo.v[CPL - 1], acc.v[CPL - 1], dsum);
                 }
             }
         }
@@ -505,8 +509,10 @@ int record_dwords(int nslot) {
     return 0;
 }
 
-// threads_a: row pass threads per block (0 = 1024); strip: strip width 64 or 128 (0 = 128);
-// ncu: compute units of the device
+// threads_a: row pass threads per block (0: resident blocks where the sector allows, else 1024);
+// strip: strip width 64 or 128 (0: 128 where the row stride ldx is even, else 64 -- the 16-byte
+// strip reads rows of odd stride at 8-byte alignment, and at the 15-site (7, 8) sector it measured
+// 426 against 403 us for 64-column strips); ncu: compute units of the device
 HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
     HvPlan p{};
     // the default shape (threads_a 0) takes the resident row pass where its records fill the
@@ -514,8 +520,8 @@ HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
     p.grid_p = (threads_a <= 0 && nU >= 2048 && nU <= kResRowsMaxU && ncu > 0 && nD >= 2 * ncu) ? ncu : 0;
     p.threads_a = threads_a > 0 ? threads_a : 1024;
     p.grid_a = nD;
-    if (strip <= 0) strip = 128;
-    p.cpl = (strip >= 128 && (ldx % 2) == 0 && (nU % 2 == 0 || nU >= 3)) ? 2 : 1;
+    if (strip <= 0) strip = (ldx % 2) == 0 ? 128 : 64;
+    p.cpl = (strip >= 128 && nU >= 2) ? 2 : 1;
     // rows per work item: 8 per wave with 128-column strips, 32 with 64-column strips (items of
     // 1-2 K elements; measured best at the 14-site (7, 7) and 15-site (7, 8) sectors)
     p.rows_b = (p.cpl == 2 ? 2 : 8) * (kStripThreads / 64);
@@ -592,16 +598,8 @@ static hipError_t hv_launch_norb(const HvArgs& a0, const HvPlan& pl, hipStream_t
     a.items_per_xcd = pl.items_per_xcd;
     hipError_t e = rows_launch<NORB>(a, pl, s);
     if (e != hipSuccess) return e;
-    const bool al16 = ((reinterpret_cast<uintptr_t>(a.x) | reinterpret_cast<uintptr_t>(a.y)) & 15) == 0;
-    if (pl.cpl == 2 && !al16) {
-        // 16-byte pairs need 16-byte aligned x and y: a caller's misaligned vectors take the
-        // 64-column strips; the x.y partials are sized for the plan's strips, so none are asked for here
-        if (a.dot_part) return hipErrorInvalidValue;
-        const HvPlan p1 = hv_plan(a.up.count, a.dn.count, pl.threads_a, 64, 0, a.ldx);
-        a.rows_b = p1.rows_b;
-        a.items_per_xcd = p1.items_per_xcd;
-        hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(p1.grid_b), dim3(kStripThreads), 0, s, a);
-    } else if (pl.cpl == 2)
+    if (((reinterpret_cast<uintptr_t>(a.x) | reinterpret_cast<uintptr_t>(a.y)) & 7) != 0) return hipErrorInvalidValue;
+    if (pl.cpl == 2)
         hipLaunchKernelGGL((hv_strip_kernel<NORB, 2>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);
     else
         hipLaunchKernelGGL((hv_strip_kernel<NORB, 1>), dim3(pl.grid_b), dim3(kStripThreads), 0, s, a);

@@ -87,7 +87,7 @@ struct HvPlan {
     int cpl;                            // strip pass: columns per lane (strip width 64 * cpl)
     int rows_b, items_per_xcd, grid_b;  // grid_b = number of x.y partials
 };
-// ldx: row stride of the vectors (nU; 128-column strips need an even ldx, and x, y 16-byte aligned)
+// ldx: row stride of the vectors (nU, or more for padded rows); x and y 8-byte aligned
 HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx);
 hipError_t launch_hv(const HvArgs& a, const HvPlan& plan, int norb, hipStream_t s);
 // record dwords the row pass reads for nslot move slots (3, 5, 7, 9, 12, 16, 24 or 32)

@@ -227,11 +227,11 @@ def test_hv_slot_counts_and_sector_shapes(case):
                                           ("ns14_roofline", (7, 7))])
 def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
     """moed_hv_device with x and y starting at either 8-byte phase (odd nU at 15 sites, where every
-    C(15, k) is odd, and even nU at 12 and 14 sites, whose 16-byte row and strip loads need 16-byte
-    aligned vectors and otherwise fall back to 8-byte loads; at 14 sites the row pass is the
-    resident one): y is bit-identical to the host-buffer product and to the 64-column (8-byte load)
-    strips in every case, and nothing is written outside y.  The device buffers come from the HIP
-    runtime the library uses."""
+    C(15, k) is odd, and even nU at 12 and 14 sites, whose row pass stages 16-byte pairs only from a
+    16-byte aligned x and otherwise single doubles, while the 128-column strips read 16-byte pairs
+    at either phase; at 14 sites the row pass is the resident one): y is bit-identical to the
+    host-buffer product and to the 64-column (8-byte load) strips in every case, and nothing is
+    written outside y.  The device buffers come from the HIP runtime the library uses."""
     import ctypes as ct
     hip = ct.CDLL("libamdhip64.so")
     hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
@@ -249,24 +249,24 @@ def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
         for b in bufs:
             assert hip.hipMalloc(ct.byref(b), nbytes) == 0
         try:
-            for xoff in (0, 1):
-                for yoff in (0, 1):
-                    print("case", name, sector, xoff, yoff, flush=True)
-                    xh = np.zeros(D + 2)
-                    xh[xoff:xoff + D] = x
-                    assert hip.hipMemcpy(bufs[0], xh.ctypes.data, nbytes, H2D) == 0
-                    assert hip.hipMemcpy(bufs[1], np.zeros(D + 2).ctypes.data, nbytes, H2D) == 0
-                    s.multiply_H_device(nu, nd, bufs[0].value + 8 * xoff, bufs[1].value + 8 * yoff)
-                    assert hip.hipDeviceSynchronize() == 0
-                    y = np.empty(D + 2)
-                    assert hip.hipMemcpy(y.ctypes.data, bufs[1], nbytes, D2H) == 0
-                    assert np.array_equal(y[yoff:yoff + D], ref)
-                    assert y[:yoff].sum() == 0.0 and y[yoff + D:].sum() == 0.0  # nothing written outside y
+            for strip in (128, 64):
+                s.set_hv_shape(0, strip)
+                for xoff in (0, 1):
+                    for yoff in (0, 1):
+                        print("case", name, sector, strip, xoff, yoff, flush=True)
+                        xh = np.zeros(D + 2)
+                        xh[xoff:xoff + D] = x
+                        assert hip.hipMemcpy(bufs[0], xh.ctypes.data, nbytes, H2D) == 0
+                        assert hip.hipMemcpy(bufs[1], np.zeros(D + 2).ctypes.data, nbytes, H2D) == 0
+                        s.multiply_H_device(nu, nd, bufs[0].value + 8 * xoff, bufs[1].value + 8 * yoff)
+                        assert hip.hipDeviceSynchronize() == 0
+                        y = np.empty(D + 2)
+                        assert hip.hipMemcpy(y.ctypes.data, bufs[1], nbytes, D2H) == 0
+                        assert np.array_equal(y[yoff:yoff + D], ref)
+                        assert y[:yoff].sum() == 0.0 and y[yoff + D:].sum() == 0.0  # nothing written outside y
         finally:
             for b in bufs:
                 hip.hipFree(b)
-        s.set_hv_shape(0, 64)
-        assert np.array_equal(s.multiply_H(nu, nd, x), ref)
 
 
 @pytest.mark.parametrize("case", [(2, 6, 4, 4), (3, 3, 3, 2), (1, 4, 0, 2), (2, 4, 6, 5)])
