@@ -69,13 +69,13 @@ inline int move_parity(uint32_t s, int p, int q) {
 }
 
 struct StrTable {
-    int n = 0, count = 0, mh = 0, nslot = 0, nq = 0;
+    int n = 0, count = 0, mh = 0, nslot = 0, rw = 0;
     std::vector<uint32_t> str;
     std::vector<int32_t> rank;  // [2^Ns]: rank within this popcount class, -1 otherwise
     DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec, d_rec8;
     StrDev dev() const {
         return {d_str.as<uint16_t>(), d_nhop.as<int32_t>(), d_mv.as<uint32_t>(), d_mvc.as<double>(),
-                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    d_rec8.as<uint4>(),  count, mh, nslot, nq, n};
+                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    d_rec8.as<uint4>(),  count, mh, nslot, rw, n};
     }
 };
 
@@ -337,10 +337,11 @@ int build_table(moed_handle* h, int n, StrTable** out) {
     }
     // row-pass records (moed_internal.h, StrDev::rec): onsite prefix, string, slot entries
     t->nslot = (int)h->slots.size();
-    // 16-byte units per record: room for the onsite prefix, the string and the slots, and for the
-    // record_dwords(nslot) dwords the row kernels read (their last slots are padding entries)
-    t->nq = std::max((10 + 2 * t->nslot + 15) / 16, (record_dwords(t->nslot) + 3) / 4);
-    const size_t ru16 = (size_t)t->nq * 8;  // uint16 per record
+    // record_dwords(nslot) dwords per record, back to back: the onsite prefix, the string and the
+    // slots (the last slot positions are padding entries when nslot is even)
+    t->rw = record_dwords(t->nslot);
+    if (t->rw == 0) return fail(h, MOED_EARG, "too many move slots per string");
+    const size_t ru16 = (size_t)t->rw * 2;  // uint16 per record
     std::vector<uint16_t> rec(ru16 * C, (uint16_t)C);
     for (int r = 0; r < C; ++r) {
         const uint32_t s = t->str[r];

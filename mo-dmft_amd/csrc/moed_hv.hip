@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     const double* __restrict__ xr = P.x + (size_t)id * P.ldx;
     double* __restrict__ yr = P.y + (size_t)id * P.ldx;
     const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(BYTES ? P.up.rec8 : P.up.rec);
-    const int rstride = 4 * P.up.nq;
+    const int rstride = P.up.rw;
     uint32_t wn[NW];
     if (tid < nU) {
 #pragma unroll
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
     const int G = gridDim.x, r0 = blockIdx.x;
     const int K = (nD - r0 + G - 1) / G;  // rows of this block: r0 + G k, k < K (K >= 1)
     const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(P.up.rec8);
-    const int rstride = 4 * P.up.nq;
+    const int rstride = P.up.rw;
     uint32_t w[kResRowsCpt][NW];
 #pragma unroll
     for (int c = 0; c < kResRowsCpt; ++c) {

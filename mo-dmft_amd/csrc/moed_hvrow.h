@@ -46,7 +46,7 @@ __device__ __forceinline__ void hv_row_compute(const BatchProbDev& d, int id, co
     const int nd = __builtin_popcount(sd);
     const cptr<double> slotc = as_const(slotcp);
     const uint32_t* __restrict__ recw = reinterpret_cast<const uint32_t*>(d.up.rec);
-    const int rstride = 4 * d.up.nq;
+    const int rstride = d.up.rw;
     const cptr<uint32_t> mvp = as_const(d.dn.mv);
     const cptr<double> mvcp = as_const(d.dn.mvc);
     const int nh = as_const(d.dn.nhop)[id];

@@ -27,7 +27,7 @@ constexpr int kXcd = 8;        // MI355X: 8 XCDs, workgroups dealt round-robin
 //                          move k of string r (k < nhop[r]): target rank and signed coefficient;
 //                          rows padded to mh (a multiple of 8) with target r and coefficient 0.0
 //   imp[(a*norb+b)*count+r] impurity move b -> a (c+_a c_b), a != b: target rank | sign << 16, or -1
-//   rec[r*nq .. r*nq+nq)   row-pass record of string r, 16*nq bytes read as nq 16-byte loads:
+//   rec[r*rw .. r*rw+rw)   row-pass record of string r, rw = record_dwords(nslot) dwords:
 //                          uint16 [0,4) the onsite prefix (sum of ek over the occupied sites in
 //                          ascending site order from 0.0, the first partial sum of multiply_H's
 //                          diagonal, ed_hamiltonian.F90:112-123) as a double; [4] the string;
@@ -54,7 +54,7 @@ struct StrDev {
     int32_t count;
     int32_t mh;
     int32_t nslot;
-    int32_t nq;
+    int32_t rw;   // dwords per row-pass record (record_dwords(nslot)), the records' stride
     int32_t nel;  // electrons per string
 };
 
@@ -90,7 +90,7 @@ struct HvPlan {
 // ldx: row stride of the vectors (nU, or more for padded rows); x and y 8-byte aligned
 HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx);
 hipError_t launch_hv(const HvArgs& a, const HvPlan& plan, int norb, hipStream_t s);
-// record dwords the row pass reads for nslot move slots (3, 5, 7, 9, 12, 16, 24 or 32)
+// record dwords the row pass reads for nslot move slots (3, 5, 7, 9, 12, 16, 24 or 32; 0 for too many)
 int record_dwords(int nslot);
 
 // ------------------------------------------------------------------ multi-problem engine (moed_eig.hip)
