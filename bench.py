@@ -529,7 +529,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes,
-                     "kernel": "one H v = hv_rows_kernel + hv_strip_kernel on one stream"},
+                     "kernel": "one H v = row pass (hv_rows_persist_kernel at this sector) + hv_strip_kernel on one stream"},
         "cpu_baseline": cpu,
         "cpu_baseline_more": cpu_extra,
         "batched_vs_plain": batched,
