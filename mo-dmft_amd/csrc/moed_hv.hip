@@ -29,6 +29,11 @@
 
 namespace moed {
 
+// two consecutive doubles as one 16-byte access; gfx950 serves it at any 8-byte alignment (the HSA
+// runtime enables unaligned access), and this type tells the compiler the address is only 8-byte
+// aligned
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+
 // ------------------------------------------------------------------ row pass
 
 // Row pass, one block per row (down string id).  The row is staged in LDS twice, as x and -x:
@@ -53,7 +58,7 @@ template <int NORB, int NW, int NDP, bool BYTES>
 __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
     constexpr int KMAX = 2 * NW - 5;
     __shared__ __attribute__((aligned(16))) double xstat[BYTES ? row_len(kFastMaxU) : 1];
-    extern __shared__ double xdyn[];
+    extern __shared__ __attribute__((aligned(16))) double xdyn[];
     double* const xs = BYTES ? xstat : xdyn;
 
     if (P.stop && *P.stop) return;
@@ -72,39 +77,30 @@ __global__ void __launch_bounds__(1024) hv_rows_kernel(HvArgs P) {
         for (int q = 0; q < NW; ++q) wn[q] = recw[(size_t)tid * rstride + q];
     }
 
-    // ---- stage the row and its negation (16-byte loads when every row starts 16-byte aligned);
-    //      each thread issues all its loads of a round before it writes any of them to LDS
-    if ((nU & 1) == 0 && (reinterpret_cast<uintptr_t>(P.x) & 15) == 0) {
+    // ---- stage the row and its negation: 16-byte pairs (rows at either 8-byte phase), the last
+    //      element of an odd row alone; each thread issues all its loads of a round before it writes
+    //      any of them to LDS
+    {
         constexpr int R = 4;
-        for (int p0 = tid; p0 < nU / 2; p0 += R * NT) {
-            double2 v[R];
+        const int np = nU / 2;
+        for (int p0 = tid; p0 < np; p0 += R * NT) {
+            d2u v[R];
 #pragma unroll
             for (int u = 0; u < R; ++u)
-                if (p0 + u * NT < nU / 2) v[u] = reinterpret_cast<const double2*>(xr)[p0 + u * NT];
+                if (p0 + u * NT < np) v[u] = reinterpret_cast<const d2u*>(xr)[p0 + u * NT];
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const int p = p0 + u * NT;
-                if (p < nU / 2) {
-                    reinterpret_cast<double2*>(xs)[p] = v[u];
+                if (p < np) {
+                    reinterpret_cast<double2*>(xs)[p] = make_double2(v[u].x, v[u].y);
                     reinterpret_cast<double2*>(xs + neg)[p] = make_double2(-v[u].x, -v[u].y);
                 }
             }
         }
-    } else {
-        constexpr int R = 8;
-        for (int k0 = tid; k0 < nU; k0 += R * NT) {
-            double v[R];
-#pragma unroll
-            for (int u = 0; u < R; ++u)
-                if (k0 + u * NT < nU) v[u] = xr[k0 + u * NT];
-#pragma unroll
-            for (int u = 0; u < R; ++u) {
-                const int k = k0 + u * NT;
-                if (k < nU) {
-                    xs[k] = v[u];
-                    xs[neg + k] = -v[u];
-                }
-            }
+        if ((nU & 1) && tid == NT - 1) {
+            const double v = xr[nU - 1];
+            xs[nU - 1] = v;
+            xs[neg + nU - 1] = -v;
         }
     }
     const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
@@ -165,60 +161,47 @@ constexpr int kResRowsBuf = 8190;  // doubles per buffer: 65,520 bytes, so the s
 constexpr int kResRowsCpt = 4;     // columns per thread: ceil(4093 / 1024)
 static_assert(row_len(kResRowsMaxU) <= kResRowsBuf, "a staged row fits its buffer");
 
-// a thread's share of one row: 16-byte pairs p = tid + 1024 j (PAIRS) or single doubles
-template <bool PAIRS>
+// a thread's share of one row: the 16-byte pairs p = tid + 1024 j (j < 2: nU <= 4093 has at most
+// 2046 pairs) and, for an odd nU, thread 1023 also the last element
 struct RowRegs {
-    double v[4];
+    d2u v[2];
+    double t;
 };
 
-template <bool PAIRS>
-__device__ __forceinline__ void persist_load(RowRegs<PAIRS>& r, const double* __restrict__ xr, int nU, int tid) {
+__device__ __forceinline__ void persist_load(RowRegs& r, const double* __restrict__ xr, int nU, int tid) {
 #pragma unroll
-    for (int j = 0; j < (PAIRS ? 2 : 4); ++j) {
-        if constexpr (PAIRS) {
-            const int p = tid + 1024 * j;
-            if (p < nU / 2) {
-                const double2 t = reinterpret_cast<const double2*>(xr)[p];
-                r.v[2 * j] = t.x;
-                r.v[2 * j + 1] = t.y;
-            }
-        } else {
-            const int k = tid + 1024 * j;
-            if (k < nU) r.v[j] = xr[k];
-        }
+    for (int j = 0; j < 2; ++j) {
+        const int p = tid + 1024 * j;
+        if (p < nU / 2) r.v[j] = reinterpret_cast<const d2u*>(xr)[p];
     }
+    if ((nU & 1) && tid == 1023) r.t = xr[nU - 1];
 }
 
-template <bool PAIRS>
-__device__ __forceinline__ void persist_store(const RowRegs<PAIRS>& r, double* xs, int nU, int tid) {
+__device__ __forceinline__ void persist_store(const RowRegs& r, double* xs, int nU, int tid) {
     const int neg = row_neg(nU);
 #pragma unroll
-    for (int j = 0; j < (PAIRS ? 2 : 4); ++j) {
-        if constexpr (PAIRS) {
-            const int p = tid + 1024 * j;
-            if (p < nU / 2) {
-                reinterpret_cast<double2*>(xs)[p] = make_double2(r.v[2 * j], r.v[2 * j + 1]);
-                reinterpret_cast<double2*>(xs + neg)[p] = make_double2(-r.v[2 * j], -r.v[2 * j + 1]);
-            }
-        } else {
-            const int k = tid + 1024 * j;
-            if (k < nU) {
-                xs[k] = r.v[j];
-                xs[neg + k] = -r.v[j];
-            }
+    for (int j = 0; j < 2; ++j) {
+        const int p = tid + 1024 * j;
+        if (p < nU / 2) {
+            reinterpret_cast<double2*>(xs)[p] = make_double2(r.v[j].x, r.v[j].y);
+            reinterpret_cast<double2*>(xs + neg)[p] = make_double2(-r.v[j].x, -r.v[j].y);
         }
+    }
+    if ((nU & 1) && tid == 1023) {
+        xs[nU - 1] = r.t;
+        xs[neg + nU - 1] = -r.t;
     }
 }
 
 // one step: stage row idn (registers -> buffer 1 - B), load row idl into registers, compute row id
 // from buffer B (idn, idl < 0: none)
-template <int NORB, int NW, int NDP, int B, bool PAIRS>
+template <int NORB, int NW, int NDP, int B>
 __device__ __forceinline__ void persist_step(const HvArgs& P, double* xb, const uint32_t (&w)[kResRowsCpt][NW],
-                                             RowRegs<PAIRS>& rr, int id, int idn, int idl) {
+                                             RowRegs& rr, int id, int idn, int idl) {
     constexpr int KMAX = 2 * NW - 5;
     const int nU = P.up.count, tid = threadIdx.x;
-    if (idn >= 0) persist_store<PAIRS>(rr, xb + (1 - B) * kResRowsBuf, nU, tid);
-    if (idl >= 0) persist_load<PAIRS>(rr, P.x + (size_t)idl * P.ldx, nU, tid);
+    if (idn >= 0) persist_store(rr, xb + (1 - B) * kResRowsBuf, nU, tid);
+    if (idl >= 0) persist_load(rr, P.x + (size_t)idl * P.ldx, nU, tid);
 
     const double U = P.U, UmJ3 = P.UmJ3, Um2J = P.Um2J;
     const uint32_t sd = scalar_u16(P.dn.str, id);
@@ -257,7 +240,7 @@ __device__ __forceinline__ void persist_step(const HvArgs& P, double* xb, const 
     __syncthreads();
 }
 
-template <int NORB, int NW, int NDP, bool PAIRS>
+template <int NORB, int NW, int NDP>
 __global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
     __shared__ __attribute__((aligned(16))) double xb[2 * kResRowsBuf];
     if (P.stop && *P.stop) return;
@@ -274,20 +257,20 @@ __global__ void __launch_bounds__(1024) hv_rows_persist_kernel(HvArgs P) {
         for (int q = 0; q < NW; ++q) w[c][q] = recw[(size_t)iu * rstride + q];
     }
     // prologue: row 0 of the block into buffer 0, row 1 into registers, the zero entries of both buffers
-    RowRegs<PAIRS> rr;
-    persist_load<PAIRS>(rr, P.x + (size_t)r0 * P.ldx, nU, tid);
-    persist_store<PAIRS>(rr, xb, nU, tid);
-    if (K > 1) persist_load<PAIRS>(rr, P.x + (size_t)(r0 + G) * P.ldx, nU, tid);
+    RowRegs rr;
+    persist_load(rr, P.x + (size_t)r0 * P.ldx, nU, tid);
+    persist_store(rr, xb, nU, tid);
+    if (K > 1) persist_load(rr, P.x + (size_t)(r0 + G) * P.ldx, nU, tid);
     if (tid < 2) {
         xb[tid * kResRowsBuf + nU] = 0.0;
         xb[tid * kResRowsBuf + row_neg(nU) + nU] = 0.0;
     }
     __syncthreads();
     for (int k = 0; k < K; k += 2) {
-        persist_step<NORB, NW, NDP, 0, PAIRS>(P, xb, w, rr, r0 + G * k, k + 1 < K ? r0 + G * (k + 1) : -1,
+        persist_step<NORB, NW, NDP, 0>(P, xb, w, rr, r0 + G * k, k + 1 < K ? r0 + G * (k + 1) : -1,
                                               k + 2 < K ? r0 + G * (k + 2) : -1);
         if (k + 1 >= K) break;
-        persist_step<NORB, NW, NDP, 1, PAIRS>(P, xb, w, rr, r0 + G * (k + 1), k + 2 < K ? r0 + G * (k + 2) : -1,
+        persist_step<NORB, NW, NDP, 1>(P, xb, w, rr, r0 + G * (k + 1), k + 2 < K ? r0 + G * (k + 2) : -1,
                                               k + 3 < K ? r0 + G * (k + 3) : -1);
     }
 }
@@ -301,7 +284,6 @@ template <int CPL>
 struct Cols {
     double v[CPL];
 };
-typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
 template <int CPL>
 __device__ __forceinline__ Cols<CPL> load_cols(const double* p) {
     Cols<CPL> c;
@@ -535,13 +517,11 @@ HvPlan hv_plan(int nU, int nD, int threads_a, int strip, int ncu, int64_t ldx) {
 
 template <int NORB, int NW, int NDP>
 static hipError_t rows_launch_nd(const HvArgs& a, const HvPlan& pl, hipStream_t s) {
-    if constexpr (NW <= 12) {
+    // where 4 columns' records and the row's diagonal terms fit the 128 registers of a 1024-thread
+    // block without spilling (larger records or more orbitals take the per-row blocks)
+    if constexpr (NW + 3 * NORB <= 15) {
         if (pl.grid_p > 0 && a.up.rec8 && a.up.count <= kResRowsMaxU) {
-            const bool pairs = (a.up.count % 2) == 0 && (a.ldx % 2) == 0 && (reinterpret_cast<uintptr_t>(a.x) & 15) == 0;
-            if (pairs)
-                hipLaunchKernelGGL((hv_rows_persist_kernel<NORB, NW, NDP, true>), dim3(pl.grid_p), dim3(1024), 0, s, a);
-            else
-                hipLaunchKernelGGL((hv_rows_persist_kernel<NORB, NW, NDP, false>), dim3(pl.grid_p), dim3(1024), 0, s, a);
+            hipLaunchKernelGGL((hv_rows_persist_kernel<NORB, NW, NDP>), dim3(pl.grid_p), dim3(1024), 0, s, a);
             return hipGetLastError();
         }
     }

@@ -142,8 +142,8 @@ def test_hv_14site_properties_and_oracle(configs):
 @pytest.mark.parametrize("sector", [(7, 7), (6, 7)])
 def test_hv_14site_resident_row_pass_equals_per_row_blocks(configs, sector):
     """the default row pass at 14 sites (resident blocks with the records in registers; nU = 3432
-    stages 16-byte pairs, nU = 3003 single doubles) against one block per row of 1024 and 512
-    threads: bit-identical; (6, 7) also against the oracle"""
+    stages 16-byte pairs only, nU = 3003 also its last element alone) against one block per row of
+    1024 and 512 threads: bit-identical; (6, 7) also against the oracle"""
     c = configs["ns14_roofline"]
     nu, nd = sector
     with EDSolver.from_config(c, tuned=False) as s:
@@ -227,11 +227,11 @@ def test_hv_slot_counts_and_sector_shapes(case):
                                           ("ns14_roofline", (7, 7))])
 def test_hv_device_vectors_at_either_8_byte_phase(name, sector):
     """moed_hv_device with x and y starting at either 8-byte phase (odd nU at 15 sites, where every
-    C(15, k) is odd, and even nU at 12 and 14 sites, whose row pass stages 16-byte pairs only from a
-    16-byte aligned x and otherwise single doubles, while the 128-column strips read 16-byte pairs
-    at either phase; at 14 sites the row pass is the resident one): y is bit-identical to the
-    host-buffer product and to the 64-column (8-byte load) strips in every case, and nothing is
-    written outside y.  The device buffers come from the HIP runtime the library uses."""
+    C(15, k) is odd, and even nU at 12 and 14 sites; the row pass stages and the 128-column strips
+    read 16-byte pairs at either phase; at 14 sites the row pass is the resident one): y is
+    bit-identical to the host-buffer product and to the 64-column (8-byte load) strips in every
+    case, and nothing is written outside y.  The device buffers come from the HIP runtime the
+    library uses."""
     import ctypes as ct
     hip = ct.CDLL("libamdhip64.so")
     hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
