@@ -143,8 +143,26 @@ __global__ void __launch_bounds__(256) lz_update_kernel(int j, const double* w, 
     const double sb = sqrt(sc.B[j - 1]);
     const double c1 = 1.0 / sb, a = sc.A[j - 1];
     const double c3 = (j >= 2) ? -(sb / sqrt(sc.B[j - 2])) : 0.0;
+    // two elements per thread and iteration as 16-byte accesses (any 8-byte alignment), the last
+    // element of an odd n by thread 0 of block 0; each element as (w - a u1) c1 + c3 u0
+    typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
     double s = 0.0;
-    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t np = n >> 1;
+    for (int64_t p = blockIdx.x * 256 + threadIdx.x; p < np; p += (int64_t)gridDim.x * 256) {
+        const d2u wv = reinterpret_cast<const d2u*>(w)[p], uv = reinterpret_cast<const d2u*>(u1)[p];
+        double v0 = (wv.x - a * uv.x) * c1, v1 = (wv.y - a * uv.y) * c1;
+        if (j >= 2) {
+            const d2u zv = reinterpret_cast<const d2u*>(u0)[p];
+            v0 += c3 * zv.x;
+            v1 += c3 * zv.y;
+        }
+        const d2u o = {v0, v1};
+        reinterpret_cast<d2u*>(out)[p] = o;
+        s += v0 * v0;
+        s += v1 * v1;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int64_t i = n - 1;
         double v = (w[i] - a * u1[i]) * c1;
         if (j >= 2) v += c3 * u0[i];
         out[i] = v;
