@@ -6,7 +6,9 @@ tests/golden/make_large.py (the oracle, SciPy's ARPACK and the reference build):
     sector and with every sector in one multi-problem run;
   * 14-site sector (7, 7): E0 and G(i w_n) of both orbitals on 2048 frequencies (200 steps);
   * 15-site three-orbital sector (7, 8) with t_ij (D = 41,409,225): the ground state as an
-    eigenpair of the oracle's Hamiltonian (Rayleigh quotient and residual);
+    eigenpair of the oracle's Hamiltonian (Rayleigh quotient and residual), E0 against ARPACK on
+    the oracle's H v, and the particle Green's function of orbital 1 (c+ psi, the reference
+    recurrence, the continued fraction on 4096 frequencies) against the oracle's chain;
   * the bath fit at the 15-site shape (nx 51, 4096 frequencies) in the reference's summation order:
     the reference build's minimization result bit for bit, with the oracle's iteration and
     FUNC-evaluation counts.
@@ -128,6 +130,37 @@ def test_ns15_ground_state_is_an_eigenpair_of_the_oracle_hamiltonian():
     assert abs(nrm2 - 1.0) <= 1e-12
     assert abs(rq - E0) <= 1e-12 * abs(E0)
     assert res <= 1e-11 * abs(E0)  # measured 1.5e-11 absolute, 4.4e-13 relative
+
+
+def test_ns15_ground_state_and_particle_green_against_arpack_and_oracle():
+    """the 15-site three-orbital sector (7, 8) with t_ij, D = 41,409,225: E0 against ARPACK on the
+    oracle's H v, then the particle chain of green_diag (ed_green.F90:136-160) for orbital 1, spin up
+    -- c+ psi into (8, 8), the reference recurrence, the continued fraction on 4096 frequencies --
+    through the library's apply_c, Lanczos and continued-fraction entry points, against the same
+    chain run by the oracle on the ARPACK ground state"""
+    z = np.load(os.path.join(GOLD, "ns15_gs.npz"))
+    c = config("ns15_fit")
+    nu, nd = c.sectors[0]
+    print(f"ns15_fit: sector ({nu}, {nd}), D = 41409225, {len(z['omega'])} frequencies", flush=True)
+    with EDSolver.from_config(c) as s:
+        kept = s.ed_solve()
+        E0 = kept[0][0]
+        print(f"  E0 gpu {E0!r} arpack {float(z['E0'])!r}", flush=True)
+        assert abs(E0 - float(z["E0"])) <= 1e-10 * abs(float(z["E0"]))
+        psi = s.eigvec(kept[0][2], kept[0][3])
+        (nuo, ndo), v = s.apply_c(nu, nd, psi, 1, 1, 1)
+        del psi
+        n2 = float(np.dot(v, v))
+        nsc, a, b = s.lanczos_iteration(nuo, ndo, v, int(z["nstep"]))
+        del v
+        b = b.copy()
+        b[0] = n2
+        gp = s.green_cf(1, E0, nsc, a, b, z["omega"])
+    assert (nuo, ndo) == (nu + 1, nd) and nsc == int(z["nsc"])
+    assert abs(n2 - float(z["normsq"])) <= 1e-10
+    e = _gerr(gp, z["gp"])
+    print(f"  G+: rel err {e:.2e}", flush=True)
+    assert e <= 1e-8
 
 
 def test_ns15_shape_fit_reference_order_bit_exact():
