@@ -72,10 +72,11 @@ struct StrTable {
     int n = 0, count = 0, mh = 0, nslot = 0, rw = 0;
     std::vector<uint32_t> str;
     std::vector<int32_t> rank;  // [2^Ns]: rank within this popcount class, -1 otherwise
-    DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec, d_rec8;
+    DevBuf d_str, d_nhop, d_mv, d_mvc, d_imp, d_rec, d_rec8, d_cperm;
     StrDev dev() const {
         return {d_str.as<uint16_t>(), d_nhop.as<int32_t>(), d_mv.as<uint32_t>(), d_mvc.as<double>(),
-                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    d_rec8.as<uint4>(),  count, mh, nslot, rw, n};
+                d_imp.as<int32_t>(),  d_rec.as<uint4>(),    d_rec8.as<uint4>(),  count, mh, nslot, rw, n,
+                d_cperm.as<int32_t>()};
     }
 };
 
@@ -313,6 +314,41 @@ int build_table(moed_handle* h, int n, StrTable** out) {
                     imp[(size_t)(a * norb + b) * C + r] = t->rank[tgt] | (move_parity(s, a, b) << 16);
                 }
             }
+    }
+    // the strings in component order under the orbital-bath moves (coefficient index below
+    // norb * nbath), components numbered by their lowest rank.  The strip pass sweeps its rows in
+    // this order, so the rows live in one XCD's L2 at a time are one component's (at 14 sites the
+    // largest holds 1225 of the 3432 rows; strip pass 54.2 -> 53.0 us).  Impurity hopping moves
+    // cross components, and with them the order measured slower (15-site sector: 402 -> 421 us),
+    // so those models keep the rank order (cperm null).
+    if (!h->has_tij) {
+        std::vector<int32_t> par(C);
+        for (int r = 0; r < C; ++r) par[r] = r;
+        auto find = [&](int32_t a) {
+            while (par[a] != a) a = par[a] = par[par[a]];
+            return a;
+        };
+        for (int r = 0; r < C; ++r)
+            for (uint32_t e : moves[r])
+                if ((int)((e >> 16) & 0xffu) < norb * nbath) {
+                    const int32_t a = find(r), b = find((int32_t)(e & 0xffffu));
+                    if (a != b) par[std::max(a, b)] = std::min(a, b);
+                }
+        std::vector<int32_t> cid(C), first(C, -1), cnt;
+        for (int r = 0; r < C; ++r) {
+            const int32_t root = find(r);
+            if (first[root] < 0) {
+                first[root] = (int32_t)cnt.size();
+                cnt.push_back(0);
+            }
+            cid[r] = first[root];
+            ++cnt[cid[r]];
+        }
+        std::vector<int32_t> off(cnt.size() + 1, 0), cperm(C);
+        for (size_t c = 0; c < cnt.size(); ++c) off[c + 1] = off[c] + cnt[c];
+        for (int r = 0; r < C; ++r) cperm[off[cid[r]]++] = r;
+        HIPCHK(t->d_cperm.ensure(C * sizeof(int32_t)));
+        HIPCHK(hipMemcpy(t->d_cperm.p, cperm.data(), C * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     // row-major move lists, padded to a multiple of 8 (the strip pass reads them eight at a time)
     const int mh = (std::max(t->mh, 1) + 7) / 8 * 8;

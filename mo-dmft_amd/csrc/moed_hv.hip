@@ -365,7 +365,8 @@ __global__ void __launch_bounds__(kStripThreads) hv_strip_kernel(HvArgs P) {
                 }
             }
         }
-        for (int id = rb * P.rows_b + wave; id < r1; id += NW) {
+        for (int pos = rb * P.rows_b + wave; pos < r1; pos += NW) {
+            const int id = P.dn.cperm ? uniform_i32(as_const(P.dn.cperm)[pos]) : pos;
             const size_t ro = (size_t)id * P.ldx;
             // ---- scalar loads: the row's first eight down moves and its down impurity moves
             const int nh = nhopc[id];

@@ -56,6 +56,11 @@ struct StrDev {
     int32_t nslot;
     int32_t rw;   // dwords per row-pass record (record_dwords(nslot)), the records' stride
     int32_t nel;  // electrons per string
+    // the strings ordered by component under the orbital-bath moves, by rank within a component:
+    // such a move keeps every orbital's count of electrons on it and its bath sites, so where each
+    // bath site couples to one orbital the strings split into classes closed under those moves.
+    // The order the strip pass sweeps rows in; null (rank order) for models with impurity hopping.
+    const int32_t* cperm;
 };
 
 // Sector H v in two passes (moed_hv.hip).
