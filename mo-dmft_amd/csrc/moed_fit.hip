@@ -10,9 +10,10 @@
 // short and the result does not depend on the budget.
 //
 // Sums over frequencies come in two orders:
-//  * reference order (default): every sum runs sequentially in the reference's loop order, one
-//    lane per sum over terms staged in LDS, so chi^2, the gradient and the CG trajectory are
-//    bit-identical to the reference build;
+//  * reference order (default): every sum gives the result of the reference's sequential loop, so
+//    chi^2, the gradient and the CG trajectory are bit-identical to the reference build.  The
+//    gradient's sums run in one lane per sum over terms staged in LDS; chi^2's sum, the one every
+//    line-search step waits for, is replayed by the whole block (moed_osum.h);
 //  * tree order (MOED_F_FIT_TREE): per-thread partial sums over a fixed frequency stride, then a
 //    fixed wave/block tree.  Deterministic and independent of the batch, but rounded differently
 //    from the reference; far less latency per evaluation.
@@ -30,6 +31,7 @@
 #include "moed_device.h"
 #include "moed_fit.h"
 #include "moed_internal.h"
+#include "moed_osum.h"
 
 namespace moed {
 namespace {
@@ -38,7 +40,8 @@ constexpr int kFitThreads = 1024;           // 16 waves: four per SIMD hide the 
 constexpr int kFitNx = 64;                    // >= nxsize; the reference's nmax is 60 (frprmn.f90:12)
 constexpr int kFitMaxBath = kMaxSite - 1;
 constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
-constexpr int kFitChunkF = kFitThreads / 4 * 3;  // frequencies per staged FUNC chunk (waves w % 4 != 0)
+constexpr int kSumSeq = 16;                    // reference-order FUNC: leading terms added one by one
+constexpr int kFitPairF = 2 * kFitThreads;     // reference-order FUNC: frequencies per chunk, two per thread
 constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
 constexpr int kGrp = 4;                        // tree order: frequencies per thread and group
 constexpr int kItmax = 20000, kBrentItmax = 100;  // frprmn.f90:12, :126
@@ -80,7 +83,9 @@ struct FitSmem {
     double nzv2[kMaxOrb][kFitMaxBath], nzv[kMaxOrb][kFitMaxBath], nzek[kMaxOrb][kFitMaxBath];
     int32_t nzk[kMaxOrb][kFitMaxBath], nnz[kMaxOrb];
     union alignas(16) {  // 16-byte aligned: chain_sum reads the staged terms in pairs
-        double fstage[2][kFitChunkF * kMaxOrb];  // FUNC terms, two chunks (reference order)
+        struct {  // reference-order FUNC: the terms of kFitPairF frequencies
+            double T[kFitPairF * kMaxOrb];
+        } os;
         double cols[kFitCols * kFitChunkG];      // gradient terms of one chunk (reference order)
         struct {  // tree order, Nw <= kFitThreads: thread t's frequency iw = t, set once per launch
             double zi[kFitThreads], rw[kFitThreads];                  // Im z, 1/w_n
@@ -98,6 +103,10 @@ struct FitSmem {
         double ek[kMaxOrb][16], v2[kMaxOrb][16], ef[kMaxOrb];
     } wp[kFitThreads / 64];
     double fred[2][kFitThreads / 64];  // tree-order FUNC: the waves' partial sums (two buffers)
+    // reference-order FUNC (ordered_sum): the waves' count totals and first ties (-1: a negative or
+    // NaN term), and the running sum and position handed back
+    double os_s, os_qk[kFitThreads / 64];
+    int32_t os_pos, os_qf[kFitThreads / 64];
     int32_t tc_mode;                   // 1: u.tc, 2: u.tw holds the frequencies of this launch
     int32_t cmd, cbuf;                 // the minimiser wave's posted command and its arguments
     double cxv;
@@ -298,8 +307,7 @@ __device__ __forceinline__ GroupResidual residual_group(const FitView& q, const 
 }
 
 // FUNC (func.f90:30-42): sum over (w, o) of |r|^2 / w_n, divided by float(Norb*Norb*Nw).
-// Reference order: waves 1.. stage the terms of frequency chunk c while lane 0 of wave 0 adds chunk
-// c - 1 to the running sum (double-buffered).  Tree order: per-thread partials, then block_sum.
+// Reference order: eval_func_ref.  Tree order: per-thread partials, then block_sum.
 template <int JN>
 __device__ __forceinline__ double func_group(const FitView& q, const FreqGroup& g, double s) {
 #pragma clang fp contract(off)
@@ -334,35 +342,174 @@ __device__ __noinline__ double eval_func_tree(const FitProb qa) {
     return S.bcast[0];
 }
 
+// s + T[0] + ... + T[n-1] in this order with one rounding per add (moed_osum.h), formed by the whole
+// block: the n <= kFitThreads no terms (no <= 2 kMaxOrb) sit in S.u.os.T, thread j's at j no .. j no + no - 1.  Every
+// thread calls it after the barrier that follows the staging, with the same s; the first nseq terms
+// are added one by one (lane 0).  Returns the sum in every thread; the terms may be restaged on
+// return.  One pass from position pos: every term's count in s's binade (sum_count), a wave scan
+// of the threads' totals (doubles: integers, exact below 2^53), the waves' totals, first ties and
+// negative/NaN flags through LDS.  If the whole rest stays in the binade with no tie, every thread
+// takes the sum from the total; otherwise the one thread whose prefix straddles the binade's limit
+// finds the first term that leaves it, the owner of that term (or of the first tie, if earlier)
+// adds it for real, and the next pass starts after it.
+__device__ __noinline__ void ordered_sum_seq(double s, int pos, int n, int nseq, bool all) {
+#pragma clang fp contract(off)
+    while (pos < n && (all || pos < nseq || !sum_normal(s))) s = s + S.u.os.T[pos++];
+    S.os_s = s;
+    S.os_pos = pos;
+}
+
+__device__ __forceinline__ double ordered_sum(double s, int n, int no, int nseq) {
+#pragma clang fp contract(off)
+    constexpr int NW = kFitThreads / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const double lim = (double)kSumLim;
+    int pos = 0;
+    while (pos < n) {
+        if (!sum_normal(s) || pos < nseq) {
+            __syncthreads();  // every thread has read the previous step's os_s and os_pos
+            if (tid == 0) ordered_sum_seq(s, pos, n, nseq, false);
+        } else {
+            const int e0 = ilogb(s);
+            double kt = 0.0;
+            int ft = n;
+            bool bad = false;
+            for (int o = 0; o < no; ++o) {
+                const int i = tid * no + o;
+                if (i >= pos && i < n) {
+                    const double t = S.u.os.T[i];
+                    bad = bad || !(t >= 0.0);
+                    bool tie;
+                    kt = kt + sum_count(t, e0, tie);
+                    if (tie) ft = min(ft, i);
+                }
+            }
+            double incl = kt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const double v = __shfl_up(incl, d, 64);
+                if (lane >= d) incl = incl + v;
+            }
+            double excl = __shfl_up(incl, 1, 64);
+            if (lane == 0) excl = 0.0;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) ft = min(ft, __shfl_xor(ft, d, 64));
+            const uint64_t bb = __ballot(bad);
+            if (lane == 63) S.os_qk[wid] = incl;
+            if (lane == 0) S.os_qf[wid] = bb != 0 ? -1 : ft;
+            __syncthreads();
+            if (tid == 0) S.os_pos = n;  // the straddling thread's find, if any (set below)
+            double woff = 0.0, tot = 0.0;
+            int ftie = n;
+            bool anybad = false;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const double t = S.os_qk[w];
+                if (w < wid) woff = woff + t;
+                tot = tot + t;
+                const int f = S.os_qf[w];
+                anybad = anybad || f < 0;
+                ftie = min(ftie, f < 0 ? n : f);
+            }
+            const double M = ldexp(s, 52 - e0), thr = lim - M;
+            if (!anybad && ftie >= n && tot <= thr) {
+                s = ldexp(M + tot, e0 - 52);
+                pos = n;
+                continue;  // block-uniform: every thread read the same totals
+            }
+            __syncthreads();  // every thread has read os_pos's reset before the find below
+            if (anybad) {
+                if (tid == 0) ordered_sum_seq(s, pos, n, nseq, true);
+            } else {
+                // the straddling thread: start <= thr < end (exact where it matters: a prefix at or
+                // below thr < 2^53 is an exact integer, and one past it stays past it when rounded)
+                const double start = woff + excl, end = woff + incl;
+                if (start <= thr && end > thr) {
+                    double c = start;
+                    for (int o = 0; o < no; ++o) {
+                        const int i = tid * no + o;
+                        if (i >= pos && i < n) {
+                            bool tie;
+                            c = c + sum_count(S.u.os.T[i], e0, tie);
+                            if (c > thr) {
+                                S.os_pos = i;
+                                break;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                const int is = min(uniform_i32(S.os_pos), ftie);  // < n
+                // the owner of term `is` adds it to the valid partial before it
+                if (tid == is / no) {
+                    double c = start;
+                    for (int o = 0; o < no; ++o) {
+                        const int i = tid * no + o;
+                        if (i >= pos && i < is) {
+                            bool tie;
+                            c = c + sum_count(S.u.os.T[i], e0, tie);
+                        }
+                    }
+                    S.os_s = ldexp(M + c, e0 - 52) + S.u.os.T[is];
+                    S.os_pos = is + 1;
+                }
+            }
+        }
+        __syncthreads();
+        s = S.os_s;
+        pos = uniform_i32(S.os_pos);
+    }
+    return s;
+}
+
+// residual() over the lists nonzero_couplings built: the divisions of four couplings issued
+// together, the adds in k order; the same operations as residual(), bit-identical
+__device__ __forceinline__ cxd residual_listed(const FitView& q, int o, int iw, double zi) {
+#pragma clang fp contract(off)
+    double re = 0.0, im = 0.0;
+    const int nn = uniform_i32(S.nnz[o]);
+    for (int n0 = 0; n0 < nn; n0 += 4) {
+        cxd t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (n0 + u < nn) t[u] = cx_rdiv(S.nzv2[o][n0 + u], {0.0 - S.nzek[o][n0 + u], zi});
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (n0 + u < nn) {
+                re = re + t[u].re;
+                im = im + t[u].im;
+            }
+    }
+    const size_t io = (size_t)o + (size_t)q.norb * iw;
+    return {(re + S.xt[q.nx - q.norb + o]) - q.dev[2 * io], im - q.dev[2 * io + 1]};
+}
+
+// FUNC in reference order (func.f90:30-42): the terms |r|^2 / w_n in the reference's (w, o) order,
+// kFitPairF frequencies at a time, each thread two neighbouring ones (their divisions independent of
+// each other, so they overlap), each chunk's added to the running sum by ordered_sum.
 __device__ __noinline__ double eval_func_ref(const FitProb qa) {
 #pragma clang fp contract(off)
     const FitView q = view(qa);
     const int tid = threadIdx.x;
     double s = 0.0;
-    {
-        // waves w % 4 != 0 stage (waves are spread over the SIMDs round-robin, so the chain's SIMD
-        // keeps no staging wave and its dependent adds do not wait behind divisions)
-        const int wid = tid >> 6, slot = (wid - 1 - (wid - 1) / 4) * 64 + (tid & 63);
-        const int nch = (q.nw + kFitChunkF - 1) / kFitChunkF;
-        for (int c = 0; c <= nch; ++c) {
-            if ((wid & 3) != 0 && c < nch) {
-                const int iw = c * kFitChunkF + slot;
-                if (iw < q.nw) {
-                    double* st = S.u.fstage[c & 1] + slot * q.norb;
-                    const double w = q.omega[iw], zi = zim(q, w);
-                    for (int o = 0; o < q.norb; ++o) {
-                        const cxd r = residual(q, S.xt, o, iw, zi);
-                        const double h = hypot_corrected(r.re, r.im);
-                        st[o] = h * h / w;
-                    }
-                }
+    nonzero_couplings(q);
+    for (int base = 0; base < q.nw; base += kFitPairF) {
+        const int iwa = base + 2 * tid, iwb = iwa + 1;
+        if (iwa < q.nw) {
+            double* st = S.u.os.T + 2 * tid * q.norb;
+            const bool hb = iwb < q.nw;
+            const double wa = q.omega[iwa], wb = hb ? q.omega[iwb] : 1.0;
+            const double za = zim(q, wa), zb = zim(q, wb);
+            for (int o = 0; o < q.norb; ++o) {
+                const cxd ra = residual_listed(q, o, iwa, za);
+                const cxd rb = residual_listed(q, o, hb ? iwb : iwa, zb);
+                const double ha = hypot_corrected(ra.re, ra.im), hbv = hypot_corrected(rb.re, rb.im);
+                st[o] = ha * ha / wa;
+                if (hb) st[q.norb + o] = hbv * hbv / wb;
             }
-            if (tid == 0 && c > 0) {
-                const int n = min(kFitChunkF, q.nw - (c - 1) * kFitChunkF) * q.norb;
-                s = chain_sum(s, S.u.fstage[(c - 1) & 1], n);
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        s = ordered_sum(s, min(kFitPairF, q.nw - base) * q.norb, 2 * q.norb, base == 0 ? kSumSeq : 0);
     }
     if (tid == 0) S.bcast[0] = s / (double)(float)(q.norb * q.norb * q.nw);
     __syncthreads();
@@ -1089,6 +1236,21 @@ __global__ void __launch_bounds__(kFitThreads) fit_eval_kernel(FitEvalArgs a) {
     }
 }
 
+// ordered_sum over n terms from s0, kFitThreads no of them at a time (nseq one by one at the start):
+// the block-wide replay checked against the sequential loop on chosen inputs
+__global__ void __launch_bounds__(kFitThreads) ordered_sum_kernel(const double* T, int n, int no, int nseq,
+                                                                   double s0, double* out) {
+    double s = s0;
+    const int chunk = kFitThreads * no;
+    for (int base = 0; base < n; base += chunk) {
+        const int m = min(chunk, n - base);
+        for (int i = threadIdx.x; i < m; i += kFitThreads) S.u.os.T[i] = T[base + i];
+        __syncthreads();
+        s = ordered_sum(s, m, no, base == 0 ? nseq : 0);
+    }
+    if (threadIdx.x == 0) out[0] = s;
+}
+
 struct Buf {
     void* p = nullptr;
     ~Buf() {
@@ -1355,3 +1517,22 @@ int fit_minimize_noalloc(const FitCtx& c, void* scratch_addr, int nfit, int nx, 
 }
 
 }  // namespace moed
+
+// s0 + terms[0] + ... + terms[n-1], one rounding per add in this order, by the block-wide replay the
+// reference-order chi^2 uses (no terms per thread, 1 <= no <= 8; the first nseq added one by one).
+// Host arrays; synchronous on the null stream.  Not part of the solver's interface: the check of the
+// replay against the plain loop on inputs chosen to exercise its fallbacks.
+extern "C" int moed_check_ordered_sum(const double* terms, int n, int no, int nseq, double s0, double* out) {
+    using namespace moed;
+    if (n < 0 || no < 1 || no > 2 * kMaxOrb || nseq < 0 || !out || (n > 0 && !terms)) return MOED_EARG;
+    FitCtx c{};
+    Buf dt, dout;
+    FCHK(dt.alloc((size_t)n * sizeof(double)));
+    FCHK(dout.alloc(sizeof(double)));
+    if (n > 0) FCHK(hipMemcpy(dt.p, terms, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kFitThreads), 0, 0, dt.as<double>(), n, no, nseq, s0,
+                       dout.as<double>());
+    FCHK(hipGetLastError());
+    FCHK(hipMemcpy(out, dout.p, sizeof(double), hipMemcpyDeviceToHost));
+    return MOED_OK;
+}

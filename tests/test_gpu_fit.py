@@ -239,3 +239,66 @@ def test_dmft_loop_steps_match_the_oracle():
             ef, ek, vk = dmft.x_to_ev(X[0], norb, nbath)
             s.set_bath(ek, vk)
             Dev = s.calc_dev(ek, vk, ef, om)
+
+
+def _sequential(s, t):
+    for v in t.tolist():
+        s = s + v
+    return s
+
+
+def test_ordered_sum_replay_equals_sequential_loop():
+    """The block-wide replay behind the reference-order chi^2 (moed_osum.h) against the plain loop
+    s = s + t(i), bit for bit, on inputs built to hit its fallbacks: exact ties (powers of two,
+    ulp-sized multiples), sums crossing many binades, jumps past the pass's binades, zeros,
+    subnormals, negative, NaN and infinite terms; several chunks of 1024 threads."""
+    import ctypes as ct
+    from moed import lib
+    L = lib()
+    f = L.moed_check_ordered_sum
+    f.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.POINTER(ct.c_double)]
+    f.restype = ct.c_int
+    rng = np.random.default_rng(7)
+    out = ct.c_double()
+    ncase = 0
+    for kind in range(10):
+        for rep in range(12):
+            no = 1 + rep % 4
+            n = int(rng.integers(1, 3 * 1024 * no + 1))
+            i = np.arange(n)
+            if kind == 0:
+                t = rng.random(n)
+            elif kind == 1:
+                t = np.ldexp(rng.random(n), -rng.integers(0, 60, n))
+            elif kind == 2:
+                t = 1.0 / (i + 1.0) ** 3
+            elif kind == 3:
+                t = np.where(rng.integers(0, 5, n) == 0, 0.0, np.ldexp(1.0, -rng.integers(0, 80, n)))
+            elif kind == 4:
+                t = np.ldexp(rng.integers(0, 7, n).astype(float), -rng.integers(50, 58, n))
+            elif kind == 5:
+                t = np.where(rng.integers(0, 50, n) == 0, np.ldexp(rng.random(n), 40), 1e-3 * rng.random(n))
+            elif kind == 6:
+                t = np.where(rng.integers(0, 200, n) == 0, -rng.random(n), rng.random(n))
+            elif kind == 7:
+                t = rng.random(n)
+                t[n // 3] = np.inf
+                if rep % 2:
+                    t[n // 2] = np.nan
+            elif kind == 8:
+                t = np.ldexp(rng.random(n), -1060)
+            else:  # chi^2-like: |r|^2 / w_n over Matsubara frequencies, a few orbitals
+                w = (2 * (i // no) + 1) * np.pi / 200.0
+                t = (rng.random(n) * 1e-3 / (1.0 + w)) ** 2 / w
+            t = np.ascontiguousarray(t, dtype=np.float64)
+            s0 = 0.0 if rep % 3 == 0 else float(np.ldexp(rng.random() + 0.5, int(rng.integers(-30, 30))))
+            if kind == 4:
+                s0 = float(2.0 ** (rep % 3))
+            nseq = 32 if rep % 2 else 0
+            rc = f(t.ctypes.data, n, no, nseq, s0, ct.byref(out))
+            assert rc == 0
+            ref = _sequential(s0, t)
+            ok = (np.isnan(ref) and np.isnan(out.value)) or np.float64(ref).tobytes() == np.float64(out.value).tobytes()
+            assert ok, (kind, rep, n, no, nseq, s0, out.value, ref)
+            ncase += 1
+    assert ncase == 120

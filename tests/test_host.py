@@ -1,6 +1,8 @@
 """Host-side logic of the package (no GPU): the DMFT loop's parameter packing and start points and
 the band structure against the oracle's restatements of the reference; the eigenvector files
 against the reference build's bytes."""
+import os
+
 import numpy as np
 import pytest
 
@@ -171,3 +173,22 @@ def test_tridiagonal_ritz_vectors_by_inverse_iteration(tmp_path):
             assert np.linalg.norm(T @ V[i] - w[i] * V[i]) <= 1e-12 * tn, (trial, kind, i)
             assert V[i][np.argmax(np.abs(V[i]))] > 0
         assert np.abs(V @ V.T - np.eye(nv)).max() <= 1e-10, (trial, kind)
+
+
+def test_ordered_sum_walk_equals_sequential_loop_on_host(tmp_path):
+    """The replay of a sequential sum behind the reference-order chi^2 (moed_osum.h): its counts and
+    lane 0's walk compiled for the host, with the workgroup's pass emulated, against s = s + t[i]
+    bit for bit on 4000 random and adversarial inputs (tests/osum_check.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("needs g++ and the ROCm headers")
+    here = os.path.dirname(__file__)
+    exe = str(tmp_path / "osum_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                    "-I" + os.path.join(here, "..", "mo-dmft_amd", "csrc"),
+                    os.path.join(here, "osum_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe, "4000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "mismatches 0" in r.stdout
