@@ -40,7 +40,7 @@ constexpr int kFitThreads = 1024;           // 16 waves: four per SIMD hide the 
 constexpr int kFitNx = 64;                    // >= nxsize; the reference's nmax is 60 (frprmn.f90:12)
 constexpr int kFitMaxBath = kMaxSite - 1;
 constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
-constexpr int kSumSeq = 16;                    // reference-order FUNC: leading terms added one by one
+constexpr int kSumSeq = 256;                   // reference-order FUNC: leading terms added by one lane
 constexpr int kFitPairF = 2 * kFitThreads;     // reference-order FUNC: frequencies per chunk, two per thread
 constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
 constexpr int kGrp = 4;                        // tree order: frequencies per thread and group
@@ -352,8 +352,15 @@ __device__ __noinline__ double eval_func_tree(const FitProb qa) {
 // takes the sum from the total; otherwise the one thread whose prefix straddles the binade's limit
 // finds the first term that leaves it, the owner of that term (or of the first tie, if earlier)
 // adds it for real, and the next pass starts after it.
+// (the leading nseq terms through chain_sum's load ring: a sum that starts small crosses a binade
+// at about every doubling of its length, and one lane adds a few hundred terms in the time of a
+// pass)
 __device__ __noinline__ void ordered_sum_seq(double s, int pos, int n, int nseq, bool all) {
 #pragma clang fp contract(off)
+    if (pos == 0 && nseq > 0) {
+        pos = min(nseq, n);
+        s = chain_sum(s, S.u.os.T, pos);
+    }
     while (pos < n && (all || pos < nseq || !sum_normal(s))) s = s + S.u.os.T[pos++];
     S.os_s = s;
     S.os_pos = pos;
@@ -398,7 +405,6 @@ __device__ __forceinline__ double ordered_sum(double s, int n, int no, int nseq)
             if (lane == 63) S.os_qk[wid] = incl;
             if (lane == 0) S.os_qf[wid] = bb != 0 ? -1 : ft;
             __syncthreads();
-            if (tid == 0) S.os_pos = n;  // the straddling thread's find, if any (set below)
             double woff = 0.0, tot = 0.0;
             int ftie = n;
             bool anybad = false;
@@ -417,13 +423,17 @@ __device__ __forceinline__ double ordered_sum(double s, int n, int no, int nseq)
                 pos = n;
                 continue;  // block-uniform: every thread read the same totals
             }
-            __syncthreads();  // every thread has read os_pos's reset before the find below
             if (anybad) {
                 if (tid == 0) ordered_sum_seq(s, pos, n, nseq, true);
             } else {
-                // the straddling thread: start <= thr < end (exact where it matters: a prefix at or
-                // below thr < 2^53 is an exact integer, and one past it stays past it when rounded)
+                // The term to add for real is the first past the binade's limit or the first tie,
+                // whichever comes first; exactly one thread owns it and knows it: the thread whose
+                // prefixes straddle the limit (start <= thr < end; exact where it matters: a prefix
+                // at or below thr < 2^53 is an exact integer, and one past it stays past it when
+                // rounded) if its crossing comes before the first tie, else the tie's owner, whose
+                // prefix before the tie is then still within the limit.
                 const double start = woff + excl, end = woff + incl;
+                int is = -1;
                 if (start <= thr && end > thr) {
                     double c = start;
                     for (int o = 0; o < no; ++o) {
@@ -432,16 +442,24 @@ __device__ __forceinline__ double ordered_sum(double s, int n, int no, int nseq)
                             bool tie;
                             c = c + sum_count(S.u.os.T[i], e0, tie);
                             if (c > thr) {
-                                S.os_pos = i;
+                                if (i < ftie) is = i;
                                 break;
                             }
                         }
                     }
                 }
-                __syncthreads();
-                const int is = min(uniform_i32(S.os_pos), ftie);  // < n
-                // the owner of term `is` adds it to the valid partial before it
-                if (tid == is / no) {
+                if (ftie < n && tid == ftie / no) {
+                    double c = start;
+                    for (int o = 0; o < no; ++o) {
+                        const int i = tid * no + o;
+                        if (i >= pos && i < ftie) {
+                            bool tie;
+                            c = c + sum_count(S.u.os.T[i], e0, tie);
+                        }
+                    }
+                    if (c <= thr) is = ftie;
+                }
+                if (is >= 0) {
                     double c = start;
                     for (int o = 0; o < no; ++o) {
                         const int i = tid * no + o;
