@@ -40,7 +40,7 @@ constexpr int kFitThreads = 1024;           // 16 waves: four per SIMD hide the 
 constexpr int kFitNx = 64;                    // >= nxsize; the reference's nmax is 60 (frprmn.f90:12)
 constexpr int kFitMaxBath = kMaxSite - 1;
 constexpr int kFitCols = 2 * kFitMaxBath + 1;  // gradient columns of one orbital pass: e_k, V(o,k), ef_o
-constexpr int kSumSeq = 256;                   // reference-order FUNC: leading terms added by one lane
+constexpr int kSumSeq = 512;                   // reference-order FUNC: leading terms added by one lane
 constexpr int kFitPairF = 2 * kFitThreads;     // reference-order FUNC: frequencies per chunk, two per thread
 constexpr int kFitChunkG = 256;                // frequencies per staged gradient chunk
 constexpr int kGrp = 4;                        // tree order: frequencies per thread and group
