@@ -342,19 +342,10 @@ __device__ __noinline__ double eval_func_tree(const FitProb qa) {
     return S.bcast[0];
 }
 
-// s + T[0] + ... + T[n-1] in this order with one rounding per add (moed_osum.h), formed by the whole
-// block: the n <= kFitThreads no terms (no <= 2 kMaxOrb) sit in S.u.os.T, thread j's at j no .. j no + no - 1.  Every
-// thread calls it after the barrier that follows the staging, with the same s; the first nseq terms
-// are added one by one (lane 0).  Returns the sum in every thread; the terms may be restaged on
-// return.  One pass from position pos: every term's count in s's binade (sum_count), a wave scan
-// of the threads' totals (doubles: integers, exact below 2^53), the waves' totals, first ties and
-// negative/NaN flags through LDS.  If the whole rest stays in the binade with no tie, every thread
-// takes the sum from the total; otherwise the one thread whose prefix straddles the binade's limit
-// finds the first term that leaves it, the owner of that term (or of the first tie, if earlier)
-// adds it for real, and the next pass starts after it.
-// (the leading nseq terms through chain_sum's load ring: a sum that starts small crosses a binade
-// at about every doubling of its length, and one lane adds a few hundred terms in the time of a
-// pass)
+// Lane 0's sequential adds for ordered_sum, leaving the running sum and the next position in S: the
+// leading nseq terms through chain_sum's load ring (a sum that starts small crosses a binade at about
+// every doubling of its length, and one lane adds a few hundred terms in the time of a pass), then
+// one by one while the sum is zero, subnormal or not finite, or (all) to the end.
 __device__ __noinline__ void ordered_sum_seq(double s, int pos, int n, int nseq, bool all) {
 #pragma clang fp contract(off)
     if (pos == 0 && nseq > 0) {
@@ -366,6 +357,16 @@ __device__ __noinline__ void ordered_sum_seq(double s, int pos, int n, int nseq,
     S.os_pos = pos;
 }
 
+// s + T[0] + ... + T[n-1] in this order with one rounding per add (moed_osum.h), formed by the whole
+// block: the n <= kFitThreads no terms (no <= 2 kMaxOrb) sit in S.u.os.T, thread j's at j no ..
+// j no + no - 1.  Every thread calls it after the barrier that follows the staging, with the same
+// s; the first nseq terms go through lane 0 (ordered_sum_seq).  Returns the sum in every thread;
+// the terms may be restaged on return.  One pass from position pos: every term's count in s's
+// binade (sum_count), a wave scan of the threads' totals (doubles: integers, exact below 2^53), the
+// waves' totals, first ties and negative/NaN flags through LDS.  If the whole rest stays in the
+// binade with no tie, every thread takes the sum from the total; otherwise the owner of the first
+// term that leaves the binade (or of the first tie, if earlier) adds it for real, and the next pass
+// starts after it.
 __device__ __forceinline__ double ordered_sum(double s, int n, int no, int nseq) {
 #pragma clang fp contract(off)
     constexpr int NW = kFitThreads / 64;
